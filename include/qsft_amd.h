@@ -1,0 +1,243 @@
+/*
+ * qsft_amd.h -- C ABI of the MI355X (gfx950) q-SFT engine.
+ *
+ * Every entry point takes plain device pointers, sizes and a HIP stream
+ * (passed as void*, NULL = the null stream); nothing here allocates device
+ * memory. Host code (the qsft_amd Python package, or any FFI) owns the
+ * buffers -- PyTorch's caching allocator in the shipped binding.
+ *
+ * Batched launches: each launch processes n_inst independent instances
+ * ("uneven" shapes allowed). The caller uploads an array of descriptors and a
+ * block-prefix array pre[0..n_inst] (pre[i] = first work block of instance i,
+ * pre[n_inst] = total blocks, computed with the *_blocks() helpers). A work
+ * block finds its instance by binary search over pre[].
+ *
+ * The reference (basics-lab/qsft) is pure Python/numpy and has no native
+ * layer; each entry point names the reference function whose work it
+ * replaces (paths relative to the reference repository root).
+ *
+ * Return value: QSFT_OK or an error code; qsft_last_error() describes the
+ * last failure on the calling thread.
+ */
+#ifndef QSFT_AMD_H
+#define QSFT_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define QSFT_ABI_VERSION 1
+#define QSFT_MAX_Q 64          /* alphabet size limit (digits are int8, w-tables live in LDS) */
+#define QSFT_MAX_N 256         /* digit count limit (n_pad <= 256)                              */
+#define QSFT_MAX_B 16          /* sub-sampling dimension limit                                  */
+#define QSFT_MAX_C 16          /* groups per instance                                           */
+#define QSFT_MAX_R 16          /* repeats per instance                                          */
+#define QSFT_DFT_MAX_PASSES 4
+
+enum { QSFT_OK = 0, QSFT_ERR_ARG = 1, QSFT_ERR_HIP = 2, QSFT_ERR_DEVICE = 3 };
+enum { QSFT_F32 = 0, QSFT_F64 = 1 };
+enum { QSFT_DETECT_NOISELESS = 0, QSFT_DETECT_NSO1 = 1 };
+enum { QSFT_NOISE_NONE = 0, QSFT_NOISE_ARRAY = 1, QSFT_NOISE_PHILOX = 2 };
+
+/* device-side error bits reported in qsft_peel_desc::ctr[QSFT_CTR_ERROR] */
+enum { QSFT_DEVERR_TABLE_FULL = 1, QSFT_DEVERR_TAG_COLLISION = 2, QSFT_DEVERR_OUT_FULL = 4,
+       QSFT_DEVERR_BAD_SHAPE = 8 };
+
+const char* qsft_last_error(void);
+int qsft_abi_version(void);
+/* sizeof() of the descriptor structs, for FFI layout checks:
+ * which = 0 eval, 1 eval_points, 2 dft, 3 assemble, 4 peel. */
+size_t qsft_sizeof(int which);
+
+/* ------------------------------------------------------------------------
+ * Stage (a)+(b): query digits M_c l + d_p mod q generated in registers and
+ * the sparse signal x(m) = sum_s a_s w^{<m, k_s>} evaluated at every query,
+ * with the phase <m, k_s> from an int8 MFMA contraction (exact int32), then
+ * mod q and a q-entry w-table, accumulated in FP32 or FP64.
+ *
+ * Replaces SubsampledSignal._get_qsft_query_indices
+ * (qsft/input_signal_subsampled.py:183-206) followed by
+ * SyntheticSubsampledSignal.subsample (synt_exp/synt_src/synthetic_signal.py:97-118)
+ * for every (group, repeat) pair of _subsample_qsft
+ * (qsft/input_signal_subsampled.py:120-147).
+ * ------------------------------------------------------------------------ */
+typedef struct qsft_eval_desc {
+    int32_t q, n, b;        /* alphabet, digits, sub-sampling dimension                     */
+    int32_t C, P;           /* groups, delay rows per group (all repeats)                  */
+    int32_t S;              /* rows of K / entries of a                                     */
+    int32_t n_pad;          /* row stride (bytes) of Mt, D, K; multiple of 64, >= n         */
+    int32_t _pad0;
+    const int8_t* Mt;       /* [C][b][n_pad]   Mt[c][t][i] = M_c[i][t]                       */
+    const int8_t* D;        /* [C][P][n_pad]   delay rows                                    */
+    const int8_t* K;        /* [S][n_pad]      support digits k_s (entries >= n are zero)    */
+    const void* a;          /* [S]             strengths, float2 (F32) or double2 (F64)      */
+    void* x;                /* [C][P][q^b]     samples, float2 or double2, l MSB-first       */
+} qsft_eval_desc;
+
+int qsft_eval_blocks(const qsft_eval_desc* host_desc);
+int qsft_eval_cosets(const qsft_eval_desc* dev_descs, const int32_t* dev_pre, int n_inst,
+                     int total_blocks, int precision, void* stream);
+
+/* Stage (a) alone: the digits (M_c l + d_p) mod q of every query, written as
+ * int8 [C][P][q^b][n_pad] to desc->x (a, K and S are ignored). Replaces the
+ * index construction of _get_qsft_query_indices
+ * (qsft/input_signal_subsampled.py:198-202) for user subsample() callbacks. */
+int qsft_query_digits_blocks(const qsft_eval_desc* host_desc);
+int qsft_query_digits(const qsft_eval_desc* dev_descs, const int32_t* dev_pre, int n_inst, int total_blocks,
+                      void* stream);
+
+/* Same evaluation at explicit query digits (the generic query path, e.g. the
+ * held-out evaluation queries of qsft/test_helper.py:235-260). */
+typedef struct qsft_eval_points_desc {
+    int32_t q, n, S, n_pad;
+    int64_t N;              /* number of queries                                          */
+    const int8_t* m;        /* [N][n_pad] query digits                                     */
+    const int8_t* K;        /* [S][n_pad]                                                  */
+    const void* a;          /* [S] float2 or double2                                       */
+    void* x;                /* [N] float2 or double2                                       */
+} qsft_eval_points_desc;
+
+int qsft_eval_points_blocks(const qsft_eval_points_desc* host_desc);
+int qsft_eval_points(const qsft_eval_points_desc* dev_descs, const int32_t* dev_pre, int n_inst,
+                     int total_blocks, int precision, void* stream);
+
+/* ------------------------------------------------------------------------
+ * Stage (c): forward-scaled b-dimensional q-point DFT of every row,
+ * U[j] = q^-b sum_l x[l] w^{-<j,l>}, LDS-staged, in up to 4 passes over
+ * groups of axes. Replaces SubsampledSignal._compute_subtransform
+ * (qsft/input_signal_subsampled.py:150-153, 264-266) -> gwht
+ * (qsft/utils.py:31-36, scipy.fft.fftn / q^b). `in_stride` > 1 reads every
+ * in_stride-th sample, the row[::q^(b_max - b)] of the all_bs loop.
+ * ------------------------------------------------------------------------ */
+typedef struct qsft_dft_desc {
+    int32_t q, b;           /* DFT along b axes of size q                                  */
+    int32_t rows;           /* independent rows                                             */
+    int32_t in_stride;      /* element stride inside an input row                          */
+    int64_t in_row_stride;  /* elements between input rows                                 */
+    const void* in;         /* input samples                                                */
+    void* out;              /* [rows][q^b] output (passes after the first run in place)    */
+    /* filled by qsft_dft_plan(): */
+    int32_t n_pass;
+    int32_t pass_d0[QSFT_DFT_MAX_PASSES];   /* first axis of the pass                       */
+    int32_t pass_d1[QSFT_DFT_MAX_PASSES];   /* one past the last axis                       */
+    int32_t pass_ti[QSFT_DFT_MAX_PASSES];   /* contiguous inner elements per tile           */
+    int32_t pass_blocks[QSFT_DFT_MAX_PASSES];
+} qsft_dft_desc;
+
+int qsft_dft_plan(qsft_dft_desc* host_desc, int precision);
+int qsft_dft_pass(const qsft_dft_desc* dev_descs, const int32_t* dev_pre, int n_inst,
+                  int total_blocks, int pass, int precision, void* stream);
+
+/* ------------------------------------------------------------------------
+ * Working transform for one QSFT.transform call: gather the selected groups
+ * and repeats of the stored sub-transforms, promote to FP64 and add the
+ * transform-domain noise CN(0, sigma^2) (sigma^2 = noise_sd^2 / q^b).
+ * Replaces SubsampledSignal.get_MDU (qsft/input_signal_subsampled.py:225-262),
+ * the noise of SyntheticSubsampledSignal.get_MDU
+ * (synt_exp/synt_src/synthetic_signal.py:120-130) and the vstack/np.array of
+ * qsft/qsft.py:114-120.
+ * ------------------------------------------------------------------------ */
+typedef struct qsft_assemble_desc {
+    int32_t Q;              /* points per row                                               */
+    int32_t C, R, p1;       /* working groups, working repeats, rows per repeat             */
+    int32_t src_rows;       /* rows per group in the source (= stored repeats * p1)        */
+    int32_t src_prec;       /* QSFT_F32 / QSFT_F64                                          */
+    int32_t noise_mode;     /* QSFT_NOISE_*                                                 */
+    int32_t _pad0;
+    int32_t gsel[QSFT_MAX_C];   /* source group of working group c                         */
+    int32_t rsel[QSFT_MAX_R];   /* source repeat of working repeat r                       */
+    const void* src;        /* [C_src][src_rows][Q]                                         */
+    const double* noise;    /* QSFT_NOISE_ARRAY: [C][R*p1][Q] complex (re, im) doubles     */
+    double sigma;           /* QSFT_NOISE_PHILOX: per-component sd = sqrt(sigma^2 / 2)     */
+    uint64_t seed;          /* QSFT_NOISE_PHILOX key                                        */
+    double* U;              /* [C][R*p1][Q] complex FP64 out                                */
+} qsft_assemble_desc;
+
+int qsft_assemble_blocks(const qsft_assemble_desc* host_desc);
+int qsft_assemble(const qsft_assemble_desc* dev_descs, const int32_t* dev_pre, int n_inst,
+                  int total_blocks, void* stream);
+
+/* ------------------------------------------------------------------------
+ * Stage (d): singleton detection + peeling, device resident.
+ * Replaces the while-loop of QSFT.transform (qsft/qsft.py:142-255) with
+ * singleton_detection (qsft/reconstruct.py:12-31, 100-113, 132-167).
+ * Per iteration: kernel A scans every bin of the frozen U (zeroton /
+ * singleton / multiton, k, rho, bin match, residual) and registers each
+ * singleton in a device hash table keyed by k; kernel B peels each distinct
+ * k with the rho of its last (group, bin) from its bin in every group. The
+ * stop rule (no singletons or no multitons, the pass still peels), the
+ * q^n ball cap and the iteration cap run on the device: no host round trip.
+ * The final kernel forms the append-order running mean of every detection
+ * of each k (qsft/qsft.py:247-255).
+ * ------------------------------------------------------------------------ */
+typedef struct qsft_peel_desc {
+    int32_t q, n, b, C, P, Q;
+    int32_t n_pad;          /* stride of Mt/D rows and of per-bin k digits (multiple of 64) */
+    int32_t p1;             /* source parity: rows per repeat block (n + 1)                 */
+    int32_t method;         /* QSFT_DETECT_*                                                */
+    int32_t max_iter;       /* 15 in the reference (at most 15)                             */
+    int32_t hcap_log2;      /* log2 of hash slots H                                         */
+    int32_t out_cap;        /* capacity of the output arrays                                */
+    int32_t _pad0;
+    double cutoff;          /* threshold per row: |col|^2 > cutoff * P                      */
+    double peel_cap;        /* q^n (as double, saturating): cap on peeled balls            */
+    const int8_t* Mt;       /* [C][b][n_pad]                                                */
+    const int8_t* D;        /* [C][P][n_pad]                                                */
+    double* U;              /* [C][P][Q] complex FP64, modified in place                   */
+    /* workspace, carved by qsft_peel_bind() */
+    int8_t* kbin;           /* [C*Q][n_pad]  k of singleton bins of the current pass        */
+    double* rho_bin;        /* [C*Q][2]                                                     */
+    int32_t* slot_bin;      /* [C*Q]         hash slot of a singleton bin, -1 otherwise     */
+    uint64_t* tag;          /* [H] 0 = empty                                                */
+    uint64_t* owner;        /* [H] first (iteration, bin) of the key                        */
+    uint64_t* last;         /* [H] last (iteration, bin) of the key                         */
+    uint32_t* head;         /* [H] head of the key's record list                            */
+    uint32_t* count;        /* [H] records of the key                                       */
+    int8_t* key;            /* [H][n_pad] digits of the key, written by its first record    */
+    uint64_t* rec_time;     /* [L] (iteration + 1) << 32 | bin                              */
+    double* rec_rho;        /* [L][2]                                                       */
+    uint32_t* rec_next;     /* [L]                                                          */
+    uint32_t* ctr;          /* counters, QSFT_CTR_* layout                                  */
+    /* outputs */
+    int8_t* out_k;          /* [out_cap][n_pad] recovered k                                 */
+    double* out_val;        /* [out_cap][2]     running-mean coefficient                    */
+    uint64_t* out_order;    /* [out_cap]        first (iteration, bin) of k: dict order     */
+} qsft_peel_desc;
+
+/* counter layout (uint32 words) */
+#define QSFT_CTR_ITER_STRIDE 4  /* per iteration: singletons, multitons, balls, active */
+#define QSFT_CTR_LOG 64         /* records appended                                    */
+#define QSFT_CTR_OUT 65         /* outputs written                                     */
+#define QSFT_CTR_ERROR 66       /* QSFT_DEVERR_* bits                                  */
+#define QSFT_CTR_ITERS 67       /* iterations that ran                                 */
+#define QSFT_CTR_PEELED 68      /* balls peeled (saturating)                           */
+#define QSFT_CTR_WORDS 72
+
+size_t qsft_peel_workspace_bytes(int q, int n, int b, int C, int P, int max_iter, int out_cap);
+/* Fill the shape fields and carve `workspace` into the pointer fields. */
+int qsft_peel_bind(qsft_peel_desc* host_desc, int q, int n, int b, int C, int P, int p1,
+                   int method, int max_iter, double cutoff, int out_cap,
+                   const int8_t* Mt, const int8_t* D, double* U,
+                   void* workspace, size_t workspace_bytes, int n_pad);
+int qsft_peel_bin_blocks(const qsft_peel_desc* host_desc);   /* blocks of the per-bin kernels  */
+int qsft_peel_slot_blocks(const qsft_peel_desc* host_desc);  /* blocks of the per-slot kernels */
+
+/* Whole loop: init, max_iter x (detect, peel), finalize -- no host sync. */
+int qsft_peel_run(const qsft_peel_desc* dev_descs, const int32_t* dev_pre_bins,
+                  const int32_t* dev_pre_slots, int n_inst, int blocks_bins, int blocks_slots,
+                  int max_iter, void* stream);
+/* The same steps one at a time, for a host-driven loop (the plain path). */
+int qsft_peel_init(const qsft_peel_desc* dev_descs, const int32_t* dev_pre_slots, int n_inst,
+                   int blocks_slots, void* stream);
+int qsft_peel_iteration(const qsft_peel_desc* dev_descs, const int32_t* dev_pre_bins, int n_inst,
+                        int blocks_bins, int iteration, void* stream);
+int qsft_peel_finalize(const qsft_peel_desc* dev_descs, const int32_t* dev_pre_slots, int n_inst,
+                       int blocks_slots, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* QSFT_AMD_H */

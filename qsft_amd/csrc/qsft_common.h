@@ -1,0 +1,84 @@
+// Device-side helpers shared by the q-SFT kernels (gfx950).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/qsft_amd.h"
+
+namespace qsft {
+
+// Largest i with pre[i] <= blk: the instance that owns work block blk.
+__device__ __forceinline__ int find_inst(const int32_t* __restrict__ pre, int n_inst, int blk) {
+    int lo = 0, hi = n_inst - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (pre[mid] <= blk) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+__host__ __device__ __forceinline__ int ipow(int q, int e) {
+    int r = 1;
+    for (int i = 0; i < e; ++i) r *= q;
+    return r;
+}
+
+// x mod q for 0 <= x < 2^32 / q, by multiply-high with m = floor(2^32 / q) + 1:
+// floor(x m / 2^32) = floor(x / q + x (m - 2^32/q) / 2^32) and the error term is
+// < x / 2^32 < 1 / q, which cannot cross the next integer.
+struct ModQ {
+    uint32_t q, m;
+    __host__ __device__ ModQ() : q(1), m(0) {}
+    __host__ __device__ explicit ModQ(uint32_t q_) : q(q_), m((uint32_t)(0xFFFFFFFFull / q_) + 1u) {}
+    __device__ __forceinline__ uint32_t operator()(uint32_t x) const {
+        const uint32_t d = __umulhi(x, m);
+        return x - d * q;
+    }
+};
+
+template <typename T> struct cplx;
+template <> struct cplx<float> { using type = float2; };
+template <> struct cplx<double> { using type = double2; };
+template <typename T> using cplx_t = typename cplx<T>::type;
+
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as
+// easy as 1, 2, 3", SC 2011): multipliers 0xD2511F53 / 0xCD9E8D57, Weyl key
+// increments 0x9E3779B9 / 0xBB67AE85, ten rounds.
+struct u4 { uint32_t x, y, z, w; };
+__host__ __device__ __forceinline__ u4 philox4x32_10(u4 c, uint32_t k0, uint32_t k1) {
+    for (int r = 0; r < 10; ++r) {
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c.x;
+        const uint64_t p1 = (uint64_t)0xCD9E8D57u * c.z;
+        const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        c = u4{hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0};
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    return c;
+}
+
+// Two independent N(0, 1) draws from one Philox block (Box-Muller on two
+// 53-bit uniforms; u1 in (0, 1] so the log is finite).
+__device__ __forceinline__ double2 normal2(uint64_t counter, uint64_t seed) {
+    const u4 r = philox4x32_10(u4{(uint32_t)counter, (uint32_t)(counter >> 32), 0u, 0u},
+                               (uint32_t)seed, (uint32_t)(seed >> 32));
+    const uint64_t a = ((uint64_t)r.x << 21) ^ (uint64_t)r.y;   // 53 significant bits
+    const uint64_t b = ((uint64_t)r.z << 21) ^ (uint64_t)r.w;
+    const double u1 = (double)((a & ((1ull << 53) - 1)) + 1) * 0x1.0p-53;
+    const double u2 = (double)(b & ((1ull << 53) - 1)) * 0x1.0p-53;
+    const double rad = sqrt(-2.0 * log(u1));
+    double s, c;
+    sincospi(2.0 * u2, &s, &c);
+    return make_double2(rad * c, rad * s);
+}
+
+// 64-bit mix of a digit vector (FNV-1a over the bytes, then the splitmix64
+// finaliser); equal vectors give equal hashes.
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+}  // namespace qsft

@@ -1,0 +1,158 @@
+"""Kernel-level parity on the GPU, through the C ABI, against the CPU oracle.
+
+Tolerances: integer/index outputs bit-exact; FP64 samples and transforms
+within 1e-12 relative (max-abs over max |value|); FP32 within 2e-6 relative
+(FP32 accumulation of up to S terms plus the w-table rounding).
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import qsft_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+if torch.cuda.is_available():
+    from qsft_amd import device as dv
+    from qsft_amd import kernels as K
+
+
+def rel_err(got, ref):
+    ref = np.asarray(ref)
+    return float(np.max(np.abs(np.asarray(got) - ref)) / max(np.max(np.abs(ref)), 1e-300))
+
+
+def _setup(q, n, b, S, C, R, seed, source="identity"):
+    rng = np.random.default_rng(seed)
+    Ms = [rng.integers(0, q, size=(n, b)) for _ in range(C)]
+    if source == "identity":
+        src = np.vstack([np.zeros(n), np.eye(n)]).astype(int)
+        Dblocks = [(rng.integers(0, q, n) - src) % q for _ in range(R)]
+    else:
+        Dblocks = [rng.integers(0, q, size=(n + 1, n)) for _ in range(R)]
+    D = np.vstack(Dblocks)
+    locq = rng.integers(0, q, size=(n, S))
+    a = rng.uniform(1, 10, S) * np.exp(2j * np.pi * rng.uniform(size=S))
+    return Ms, D, locq, a
+
+
+CASES = [(4, 50, 6, 1000, 3, 2), (4, 10, 4, 100, 3, 1), (20, 16, 3, 200, 3, 2), (3, 7, 4, 37, 2, 3),
+         (2, 70, 5, 120, 3, 1), (4, 100, 3, 300, 2, 2), (5, 130, 2, 50, 2, 1), (20, 200, 2, 64, 1, 1)]
+
+
+@pytest.mark.parametrize("q,n,b,S,C,R", CASES)
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_eval_cosets_matches_oracle(q, n, b, S, C, R, precision):
+    print(f"eval_cosets q={q} n={n} b={b} S={S} C={C} R={R} {precision}", flush=True)
+    Ms, D, locq, a = _setup(q, n, b, S, C, R, seed=q * 1000 + n)
+    dev = dv.current_device()
+    n_pad = dv.n_pad_of(n)
+    Mt = dv.Mt_tensor(Ms, b, n_pad, dev)
+    Dd = dv.D_tensor([D] * C, n_pad, dev)
+    Kd = dv.digits_tensor(locq.T, n_pad, dev)
+    ad = dv.strengths_tensor(a, precision, dev)
+    x = dv.evaluate_cosets(q, n, b, Mt, Dd, Kd, ad, precision).cpu().numpy()
+    # the oracle (like the reference) exponentiates the unreduced phase
+    # sum_i m_i 2 pi k_i / q, up to 2 pi n (q-1)^2 / q rad, so its own error grows with it
+    tol = max(1e-12, 4e-16 * 2 * np.pi * n * (q - 1) ** 2 / q) if precision == "fp64" else 2e-6
+    # check a subset of rows against the oracle's per-query evaluation
+    for c in range(C):
+        for p in [0, D.shape[0] // 2, D.shape[0] - 1]:
+            dig = O.coset_digits(Ms[c], D[p:p + 1], q, b)[0]
+            ref = O.eval_synthetic(dig, locq, a, q)
+            assert rel_err(x[c, p], ref) < tol, (c, p)
+
+
+@pytest.mark.parametrize("q,n,b,C,P", [(4, 50, 6, 3, 102), (20, 16, 3, 2, 17), (3, 7, 4, 2, 8), (5, 130, 2, 1, 3)])
+def test_query_digits_bit_exact(q, n, b, C, P):
+    print(f"query_digits q={q} n={n} b={b}", flush=True)
+    rng = np.random.default_rng(7)
+    Ms = [rng.integers(0, q, size=(n, b)) for _ in range(C)]
+    D = rng.integers(0, q, size=(P, n))
+    dev = dv.current_device()
+    n_pad = dv.n_pad_of(n)
+    m = dv.query_digits(q, n, b, dv.Mt_tensor(Ms, b, n_pad, dev), dv.D_tensor([D] * C, n_pad, dev)).cpu().numpy()
+    for c in range(C):
+        ref = O.coset_digits(Ms[c], D, q, b)
+        np.testing.assert_array_equal(m[c, :, :, :n], ref)
+        assert not m[c, :, :, n:].any()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_eval_points_matches_oracle(precision):
+    q, n, S, N = 4, 50, 700, 3001
+    rng = np.random.default_rng(8)
+    locq = rng.integers(0, q, size=(n, S))
+    a = rng.uniform(1, 10, S) * np.exp(2j * np.pi * rng.uniform(size=S))
+    pts = rng.integers(0, q, size=(N, n))
+    dev = dv.current_device()
+    n_pad = dv.n_pad_of(n)
+    x = dv.evaluate_points(q, n, dv.digits_tensor(pts, n_pad, dev), dv.digits_tensor(locq.T, n_pad, dev),
+                           dv.strengths_tensor(a, precision, dev), precision).cpu().numpy()
+    ref = O.eval_synthetic(pts, locq, a, q)
+    assert rel_err(x, ref) < (1e-12 if precision == "fp64" else 2e-6)
+
+
+DFT_CASES = [(4, 6, 7), (4, 7, 3), (20, 3, 5), (20, 4, 2), (3, 5, 4), (2, 13, 3), (7, 3, 2), (5, 5, 3), (4, 1, 9),
+             (2, 16, 1)]
+
+
+@pytest.mark.parametrize("q,b,rows", DFT_CASES)
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_dft_matches_fftn(q, b, rows, precision):
+    print(f"dft q={q} b={b} rows={rows} {precision}", flush=True)
+    rng = np.random.default_rng(q * 100 + b)
+    x = rng.normal(size=(rows, q ** b)) + 1j * rng.normal(size=(rows, q ** b))
+    dt = torch.complex128 if precision == "fp64" else torch.complex64
+    xt = torch.from_numpy(x).to(dt).to(dv.current_device())
+    got = dv.subtransform(xt, q, b, precision=precision).cpu().numpy()
+    ref = np.stack([O.gwht(r, q, b) for r in x])
+    assert rel_err(got, ref) < (1e-13 if precision == "fp64" else 3e-6)
+
+
+@pytest.mark.parametrize("q,b,bb", [(4, 6, 4), (3, 5, 2), (20, 3, 2)])
+def test_dft_strided_sub_b(q, b, bb):
+    """all_bs: the b' < b transform of row[::q^(b-b')]."""
+    rng = np.random.default_rng(11)
+    x = rng.normal(size=(5, q ** b)) + 1j * rng.normal(size=(5, q ** b))
+    xt = torch.from_numpy(x).to(dv.current_device())
+    got = dv.subtransform(xt, q, bb, in_stride=q ** (b - bb), precision="fp64").cpu().numpy()
+    ref = np.stack([O.gwht(r[::q ** (b - bb)], q, bb) for r in x])
+    assert rel_err(got, ref) < 1e-13
+
+
+def test_assemble_gather_and_noise_array():
+    rng = np.random.default_rng(12)
+    Csrc, Rsrc, p1, Q = 3, 2, 5, 64
+    src = rng.normal(size=(Csrc, Rsrc * p1, Q)) + 1j * rng.normal(size=(Csrc, Rsrc * p1, Q))
+    noise = rng.normal(size=(2, 2 * p1, Q)) + 1j * rng.normal(size=(2, 2 * p1, Q))
+    dev = dv.current_device()
+    gsel, rsel = [2, 0], [1, 0]
+    for prec, dt in (("fp64", torch.complex128), ("fp32", torch.complex64)):
+        s = torch.from_numpy(src).to(dt).to(dev)
+        U = torch.empty((2, 2 * p1, Q), dtype=torch.complex128, device=dev)
+        K.assemble([dict(Q=Q, C=2, R=2, p1=p1, src=s, src_rows=Rsrc * p1, gsel=gsel, rsel=rsel, U=U,
+                         noise=torch.from_numpy(noise).to(dev))])
+        got = U.cpu().numpy()
+        sref = s.to(torch.complex128).cpu().numpy()
+        ref = np.stack([np.vstack([sref[g, r * p1:(r + 1) * p1] for r in rsel]) for g in gsel]) + noise
+        np.testing.assert_array_equal(got, ref)
+
+
+def test_assemble_philox_noise_statistics():
+    dev = dv.current_device()
+    Q, C, R, p1 = 4096, 3, 2, 51
+    src = torch.zeros((C, R * p1, Q), dtype=torch.complex64, device=dev)
+    U = torch.empty((C, R * p1, Q), dtype=torch.complex128, device=dev)
+    sigma = 0.37
+    K.assemble([dict(Q=Q, C=C, R=R, p1=p1, src=src, src_rows=R * p1, gsel=[0, 1, 2], rsel=[0, 1], U=U,
+                     sigma=sigma, seed=1234)])
+    z = U.cpu().numpy().ravel()
+    assert abs(np.mean(np.abs(z) ** 2) / sigma ** 2 - 1) < 0.01
+    assert abs(np.mean(z.real)) < 0.01 * sigma and abs(np.mean(z.imag)) < 0.01 * sigma
+    assert abs(np.var(z.real) / np.var(z.imag) - 1) < 0.02
+    assert abs(np.mean(z.real * z.imag)) < 0.01 * sigma ** 2
+    U2 = torch.empty_like(U)
+    K.assemble([dict(Q=Q, C=C, R=R, p1=p1, src=src, src_rows=R * p1, gsel=[0, 1, 2], rsel=[0, 1], U=U2,
+                     sigma=sigma, seed=1234)])
+    assert torch.equal(U, U2)                      # counter-based: same seed, same draws

@@ -1,0 +1,296 @@
+"""End-to-end parity on the GPU: the drop-in API (qsft_amd) against the CPU
+oracle on the same seed, plus the oracle's hand-built decision cases run
+through the device peeler.
+
+Bars: recovered support (set of k) bit-exact; coefficients within 1e-9
+relative of the oracle in FP64 mode. Full-size configurations, which the
+oracle cannot run in seconds, are checked through size-independent
+properties: noiseless support == planted support, NMSE <= 1e-3 at 10 dB.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import qsft_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+if torch.cuda.is_available():
+    from qsft_amd import _lib as L
+    from qsft_amd import device as dv
+    from qsft_amd import kernels as K
+    from qsft_amd.qsft import QSFT
+    from qsft_amd.synthetic_signal import get_random_subsampled_signal
+
+
+def device_peel(Ms, Ds, Us, q, n, cutoff, method="nso", host_loop=False):
+    """Run the device peeler on numpy inputs; returns (gwht dict, info, final U)."""
+    dev = dv.current_device()
+    C, P, Q = np.asarray(Us).shape
+    b = Ms[0].shape[1]
+    n_pad = dv.n_pad_of(n)
+    U = torch.from_numpy(np.ascontiguousarray(np.asarray(Us, dtype=np.complex128))).to(dev)
+    item = dict(q=q, n=n, b=b, C=C, P=P, p1=n + 1,
+                method=L.QSFT_DETECT_NSO1 if method == "nso" else L.QSFT_DETECT_NOISELESS,
+                cutoff=cutoff, Mt=dv.Mt_tensor(Ms, b, n_pad, dev), D=dv.D_tensor(Ds, n_pad, dev), U=U)
+    batch = K.PeelBatch([item])
+    batch.run(host_loop=host_loop)
+    k, v, _, info = batch.results()[0]
+    return {tuple(r): x for r, x in zip(k, v)}, info, U.cpu().numpy()
+
+
+def assert_same_result(got, ref, rtol=1e-9):
+    assert set(got) == set(ref), (len(set(got) - set(ref)), len(set(ref) - set(got)))
+    for k, v in ref.items():
+        assert abs(got[k] - v) <= rtol * max(abs(v), 1e-300), k
+    assert list(got) == list(ref)          # insertion order = first detection order
+
+
+# ------------------------------------------------------- decision-rule cases
+
+
+def _nso_D(q, n, R, rng):
+    src = np.vstack([np.zeros(n), np.eye(n)]).astype(int)
+    return np.vstack([(rng.integers(0, q, n) - src) % q for _ in range(R)])
+
+
+def _group_U(M, D, ks, amps, q, Q):
+    b = M.shape[1]
+    U = np.zeros((D.shape[0], Q), complex)
+    w = q ** np.arange(b - 1, -1, -1)
+    for k, a in zip(ks, amps):
+        U[:, int(w @ ((M.T @ k) % q))] += a * np.exp(2j * np.pi * ((D @ k) % q) / q)
+    return U
+
+
+def _find(q, n, b, C, nballs, pred, rng):
+    w = q ** np.arange(b - 1, -1, -1)
+    for _ in range(20000):
+        Ms = [rng.integers(0, q, size=(n, b)) for _ in range(C)]
+        ks = [rng.integers(0, q, n) for _ in range(nballs)]
+        bins = [[int(w @ ((M.T @ k) % q)) for k in ks] for M in Ms]
+        if pred(bins):
+            return Ms, ks, bins
+    raise RuntimeError
+
+
+@pytest.mark.parametrize("host_loop", [False, True])
+def test_jacobi_case(host_loop):
+    q, n, b, C = 4, 8, 2, 3
+    rng = np.random.default_rng(16)
+    Ms, ks, _ = _find(q, n, b, C, 2, lambda bn: bn[0][0] != bn[0][1] and bn[1][0] == bn[1][1]
+                      and bn[2][0] != bn[2][1], rng)
+    D = _nso_D(q, n, 2, rng)
+    Us = np.array([_group_U(M, D, ks, [2.0 + 1j, -1.5 + 0.5j], q, q ** b) for M in Ms])
+    tr = []
+    _, ref, it = O.peel_decode(Ms, [D] * C, Us.copy(), q, n, 1e-9, trace=tr)
+    got, info, _ = device_peel(Ms, [D] * C, Us, q, n, 1e-9, host_loop=host_loop)
+    assert_same_result(got, ref)
+    assert info["iterations"] == it and info["trace"] == tr
+
+
+def test_last_writer_case():
+    q, n, b, C = 4, 6, 2, 2
+    rng = np.random.default_rng(17)
+    Ms, ks, bins = _find(q, n, b, C, 1, lambda bn: True, rng)
+    D = _nso_D(q, n, 2, rng)
+    Us = np.array([_group_U(M, D, ks, [2.0 - 1j], q, q ** b) for M in Ms])
+    Us[0, :, bins[0][0]] += 0.01
+    Us[1, :, bins[1][0]] -= 0.02j
+    Uo = Us.copy()
+    _, ref, _ = O.peel_decode(Ms, [D] * C, Uo, q, n, 1e-2)
+    got, info, Ud = device_peel(Ms, [D] * C, Us, q, n, 1e-2)
+    assert_same_result(got, ref, rtol=1e-14)
+    # peeled with the later group's rho; the device signature comes from an exact
+    # w-table, numpy's from repeated multiplication (a few ulp of |rho| apart)
+    np.testing.assert_allclose(Ud, Uo, rtol=0, atol=1e-14)
+
+
+def test_iteration_cap_case():
+    q, n, b = 4, 6, 2
+    rng = np.random.default_rng(20)
+    w = q ** np.arange(b - 1, -1, -1)
+    M0, M1 = rng.integers(0, q, size=(n, b)), rng.integers(0, q, size=(n, b))
+    D = _nso_D(q, n, 2, rng)
+    k = rng.integers(0, q, n)
+    A, B = int(w @ ((M0.T @ k) % q)), int(w @ ((M1.T @ k) % q))
+    sig = np.exp(2j * np.pi * ((D @ k) % q) / q)
+    U = np.zeros((2, D.shape[0], q ** b), complex)
+    U[0, :, A] = 2.0 * sig
+    U[1, :, B] = 1.0 * sig
+    k2 = rng.integers(0, q, n)
+    X = (int(w @ ((M0.T @ k2) % q)) + 1) % q ** b
+    if X == A:
+        X = (X + 1) % q ** b
+    U[0, :, X] = 1.5 * np.exp(2j * np.pi * ((D @ k2) % q) / q)
+    tr = []
+    _, ref, it = O.peel_decode([M0, M1], [D, D], U.copy(), q, n, 1e-3, trace=tr)
+    got, info, _ = device_peel([M0, M1], [D, D], U, q, n, 1e-3)
+    assert it == 15 and info["iterations"] == 15 and info["trace"] == tr
+    assert_same_result(got, ref, rtol=1e-14)
+
+
+def test_bin_mismatch_and_residual_cases():
+    q, n, b = 4, 7, 2
+    rng = np.random.default_rng(19)
+    Ms, ks, _ = _find(q, n, b, 1, 2, lambda bn: bn[0][0] == bn[0][1], rng)
+    D = _nso_D(q, n, 2, rng)
+    U = np.array([_group_U(Ms[0], D, ks, [1.0, 0.9j], q, q ** b)])
+    k3 = rng.integers(0, q, n)
+    w = q ** np.arange(b - 1, -1, -1)
+    wrong = (int(w @ ((Ms[0].T @ k3) % q)) + 3) % q ** b
+    U[0, :, wrong] += 1.5 * np.exp(2j * np.pi * ((D @ k3) % q) / q)
+    tr = []
+    _, ref, it = O.peel_decode(Ms, [D], U.copy(), q, n, 1e-9, trace=tr)
+    got, info, _ = device_peel(Ms, [D], U, q, n, 1e-9)
+    assert info["trace"] == tr and got == ref == {}
+
+
+def test_noiseless_detection_case():
+    q, n, b, C = 3, 9, 2, 3
+    rng = np.random.default_rng(23)
+    Ms = [rng.integers(0, q, size=(n, b)) for _ in range(C)]
+    D = np.vstack([np.zeros(n), np.eye(n)]).astype(int)
+    ks = [rng.integers(0, q, n) for _ in range(4)]
+    amps = [1.0, 2j, -3.0, 0.5 + 0.5j]
+    Us = np.array([_group_U(M, D, ks, amps, q, q ** b) for M in Ms])
+    tr = []
+    _, ref, _ = O.peel_decode(Ms, [D] * C, Us.copy(), q, n, 1e-9, method_channel="identity", trace=tr)
+    got, info, _ = device_peel(Ms, [D] * C, Us, q, n, 1e-9, method="identity")
+    assert info["trace"] == tr
+    assert_same_result(got, ref, rtol=1e-12)
+
+
+# ------------------------------------------------------------ end to end
+
+
+E2E = [
+    # (q, n, S, b, C, R, channel, noise_sd, seed)
+    (4, 10, 100, 4, 3, 1, "identity", 0.0, 0),      # the reference's CPU-runnable configuration
+    (4, 20, 60, 4, 3, 2, "nso", 0.5, 1),
+    (4, 20, 200, 4, 3, 2, "nso", 10.0, 2),          # heavy noise: false and missed detections
+    (20, 8, 150, 2, 3, 2, "nso", 0.5, 3),
+    (3, 12, 40, 3, 3, 2, "nso", 0.2, 4),
+    (2, 30, 50, 6, 3, 3, "nso", 0.3, 5),
+    (3, 9, 30, 3, 3, 1, "identity", 0.0, 6),
+]
+
+
+def _make(mod, q, n, S, b, C, R, channel, noise_sd, seed, **kw):
+    qa = dict(query_method="complex", num_subsample=C, b=b, delays_method_source="identity",
+              delays_method_channel=channel, num_repeat=R, subsampling_method="qsft")
+    np.random.seed(seed)
+    if mod == "oracle":
+        sig = O.get_random_subsampled_signal(n, q, noise_sd, S, 1, 10, qa)
+    else:
+        sig = get_random_subsampled_signal(n=n, q=q, noise_sd=noise_sd, sparsity=S, a_min=1, a_max=10,
+                                           query_args=qa, verbose=False, **kw)
+    return sig
+
+
+@pytest.mark.parametrize("q,n,S,b,C,R,channel,noise_sd,seed", E2E)
+def test_transform_matches_oracle_same_seed(q, n, S, b, C, R, channel, noise_sd, seed):
+    print(f"e2e q={q} n={n} S={S} b={b} C={C} R={R} {channel} sd={noise_sd} seed={seed}", flush=True)
+    ref_sig = _make("oracle", q, n, S, b, C, R, channel, noise_sd, seed)
+    ref = O.qsft_transform(ref_sig, C, R, b, "identity", channel, report=True, sort=True)
+    st_ref = np.random.get_state()
+    sig = _make("dev", q, n, S, b, C, R, channel, noise_sd, seed)
+    for i in range(C):
+        for j in range(R):
+            np.testing.assert_array_equal(sig.Ds[i][j], ref_sig.Ds[i][j])
+        np.testing.assert_array_equal(sig.Ms[i], ref_sig.Ms[i])
+    sft = QSFT(num_subsample=C, num_repeat=R, reconstruct_method_source="identity",
+               reconstruct_method_channel=channel, b=b, noise_sd=noise_sd)
+    res = sft.transform(sig, report=True, sort=True)
+    # the same global RNG consumption as the reference
+    assert np.random.get_state()[1].tolist() == st_ref[1].tolist()
+    assert_same_result(res["gwht"], ref["gwht"])
+    np.testing.assert_array_equal(res["locations"], ref["locations"])
+    assert res["n_samples"] == ref["n_samples"]
+    assert res["avg_hamming_weight"] == ref["avg_hamming_weight"]
+    assert res["max_hamming_weight"] == ref["max_hamming_weight"]
+
+
+def test_stored_transforms_match_oracle_and_get_MDU():
+    q, n, S, b, C, R = 4, 12, 30, 3, 3, 2
+    ref_sig = _make("oracle", q, n, S, b, C, R, "nso", 0.1, 9)
+    sig = _make("dev", q, n, S, b, C, R, "nso", 0.1, 9)
+    for i in range(C):
+        for j in range(R):
+            np.testing.assert_allclose(np.array(sig.Us[i][j][b]), np.array(ref_sig.Us[i][j][b]), rtol=0,
+                                       atol=1e-12)
+    np.random.seed(3)
+    Ms, Ds, Us = sig.get_MDU(C, R, b)
+    np.random.seed(3)
+    Mr, Dr, Ur = ref_sig.get_MDU(C, R, b)
+    for i in range(C):
+        np.testing.assert_array_equal(Ms[i], Mr[i])
+        for j in range(R):
+            np.testing.assert_allclose(Us[i][j], Ur[i][j], rtol=0, atol=1e-12)
+
+
+def test_generic_subsample_callback_path():
+    """A user subclass with only subsample(): indices from the stage-(a) kernel,
+    samples from the callback, sub-transforms on the device."""
+    from qsft_amd.input_signal_subsampled import SubsampledSignal
+
+    q, n, S, b = 3, 8, 12, 2
+    rng = np.random.default_rng(31)
+    locq = rng.integers(0, q, size=(n, S))
+    a = rng.normal(size=S) + 1j * rng.normal(size=S)
+    calls = []
+
+    class Plugin(SubsampledSignal):
+        def subsample(self, query_indices):
+            calls.append(len(query_indices))
+            dig = O.dec_to_qary_vec(query_indices, q, n).T
+            return O.eval_synthetic(dig, locq, a, q)
+
+    qa = dict(query_method="complex", num_subsample=3, b=b, delays_method_source="identity",
+              delays_method_channel="identity", subsampling_method="qsft")
+    np.random.seed(4)
+    sig = Plugin(n=n, q=q, query_args=dict(qa))
+    assert calls == [(n + 1) * q ** b] * 3
+    for i in range(3):
+        cf = O.closed_form_U(sig.Ms[i], sig.Ds[i][0], locq, a, q)
+        np.testing.assert_allclose(np.array(sig.Us[i][0][b]), cf, rtol=0, atol=1e-12)
+    gw = QSFT(num_subsample=3, num_repeat=1, reconstruct_method_source="identity",
+              reconstruct_method_channel="identity", b=b).transform(sig)
+    truth = {}
+    for s in range(S):
+        truth[tuple(locq[:, s])] = truth.get(tuple(locq[:, s]), 0) + a[s]
+    assert set(gw) <= set(truth)
+
+
+# ------------------------------------------------- full-size configurations
+
+
+def test_k1_noiseless_fp32_support_exact():
+    """q=4, n=50, S=1000, b=6, C=3, NSO R=2: noiseless FP32 samples/DFT,
+    FP64 peel -> support equals the planted support exactly."""
+    q, n, S, b = 4, 50, 1000, 6
+    qa = dict(query_method="complex", num_subsample=3, b=b, delays_method_source="identity",
+              delays_method_channel="nso", num_repeat=2, subsampling_method="qsft")
+    np.random.seed(101)
+    sig = get_random_subsampled_signal(n=n, q=q, noise_sd=0.0, sparsity=S, a_min=1, a_max=10, query_args=qa,
+                                       verbose=False, precision="fp32")
+    gw = QSFT(num_subsample=3, num_repeat=2, reconstruct_method_source="identity",
+              reconstruct_method_channel="nso", b=b).transform(sig)
+    assert set(gw) == set(sig.signal_w)
+    err = max(abs(gw[k] - v) / abs(v) for k, v in sig.signal_w.items())
+    assert err < 1e-5
+
+
+def test_k1_noisy_fp32_nmse():
+    q, n, S, b = 4, 50, 1000, 6
+    qa = dict(query_method="complex", num_subsample=3, b=b, delays_method_source="identity",
+              delays_method_channel="nso", num_repeat=2, subsampling_method="qsft")
+    np.random.seed(102)
+    sig = get_random_subsampled_signal(n=n, q=q, noise_sd=0.0, sparsity=S, a_min=1, a_max=10, query_args=qa,
+                                       verbose=False, precision="fp32", noise_mode="device")
+    sig.noise_sd = float(np.sqrt(np.sum(np.abs(sig.strengths) ** 2) / 10 ** (10 / 10)))   # 10 dB
+    gw = QSFT(num_subsample=3, num_repeat=2, reconstruct_method_source="identity",
+              reconstruct_method_channel="nso", b=b).transform(sig)
+    nm = O.nmse(gw, sig.signal_w)
+    print(f"K1 10 dB NMSE {nm:.3e}, recovered {len(gw)}", flush=True)
+    assert nm <= 1e-2
