@@ -157,8 +157,10 @@ typedef struct qsft_assemble_desc {
 } qsft_assemble_desc;
 
 int qsft_assemble_blocks(const qsft_assemble_desc* host_desc);
+/* seed_offset is added to every descriptor's Philox seed, so one uploaded
+ * batch can draw fresh noise on every launch. */
 int qsft_assemble(const qsft_assemble_desc* dev_descs, const int32_t* dev_pre, int n_inst,
-                  int total_blocks, void* stream);
+                  int total_blocks, uint64_t seed_offset, void* stream);
 
 /* ------------------------------------------------------------------------
  * Stage (d): singleton detection + peeling, device resident.
@@ -236,6 +238,17 @@ int qsft_peel_iteration(const qsft_peel_desc* dev_descs, const int32_t* dev_pre_
                         int blocks_bins, int iteration, void* stream);
 int qsft_peel_finalize(const qsft_peel_desc* dev_descs, const int32_t* dev_pre_slots, int n_inst,
                        int blocks_slots, void* stream);
+
+/* Gather every instance's recovered rows into one buffer `dst` (device or
+ * host-mapped) so a batch comes back in one copy: an int64 header of
+ * qsft_peel_pack_header_bytes(n_inst) bytes holding counts[n_inst],
+ * offsets[n_inst], total, error bits; then rows of row_stride bytes:
+ * k digits (the instance's n_pad bytes), value (2 doubles), first detection
+ * (uint64). Rows beyond cap_rows are dropped and flagged QSFT_DEVERR_OUT_FULL.
+ * Replaces the result dict assembly of qsft/qsft.py:247-255. */
+size_t qsft_peel_pack_header_bytes(int n_inst);
+int qsft_peel_pack(const qsft_peel_desc* dev_descs, int n_inst, void* dst, int64_t cap_rows, int row_stride,
+                   void* stream);
 
 #ifdef __cplusplus
 }

@@ -82,7 +82,7 @@ SIGNATURES = {
     "qsft_dft_plan": (ctypes.c_int, [ctypes.POINTER(DftDesc), ctypes.c_int]),
     "qsft_dft_pass": (ctypes.c_int, [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp]),
     "qsft_assemble_blocks": (ctypes.c_int, [ctypes.POINTER(AssembleDesc)]),
-    "qsft_assemble": (ctypes.c_int, [vp, vp, ctypes.c_int, ctypes.c_int, vp]),
+    "qsft_assemble": (ctypes.c_int, [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_uint64, vp]),
     "qsft_peel_workspace_bytes": (ctypes.c_size_t, [ctypes.c_int] * 7),
     "qsft_peel_bind": (ctypes.c_int, [ctypes.POINTER(PeelDesc), ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                       ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
@@ -94,6 +94,8 @@ SIGNATURES = {
     "qsft_peel_init": (ctypes.c_int, [vp, vp, ctypes.c_int, ctypes.c_int, vp]),
     "qsft_peel_iteration": (ctypes.c_int, [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp]),
     "qsft_peel_finalize": (ctypes.c_int, [vp, vp, ctypes.c_int, ctypes.c_int, vp]),
+    "qsft_peel_pack_header_bytes": (ctypes.c_size_t, [ctypes.c_int]),
+    "qsft_peel_pack": (ctypes.c_int, [vp, ctypes.c_int, vp, ctypes.c_int64, ctypes.c_int, vp]),
 }
 
 

@@ -11,7 +11,8 @@ constexpr int kAsmThreads = 256;
 constexpr int kAsmPerThread = 4;
 
 __global__ __launch_bounds__(kAsmThreads) void k_assemble(const qsft_assemble_desc* __restrict__ descs,
-                                                          const int32_t* __restrict__ pre, int n_inst) {
+                                                          const int32_t* __restrict__ pre, int n_inst,
+                                                          uint64_t seed_offset) {
     const int inst = find_inst(pre, n_inst, blockIdx.x);
     const qsft_assemble_desc& d = descs[inst];
     const int64_t Q = d.Q;
@@ -43,7 +44,7 @@ __global__ __launch_bounds__(kAsmThreads) void k_assemble(const qsft_assemble_de
             vx = vx + z.x;
             vy = vy + z.y;
         } else if (d.noise_mode == QSFT_NOISE_PHILOX) {
-            const double2 z = normal2((uint64_t)e, d.seed);
+            const double2 z = normal2((uint64_t)e, d.seed + seed_offset);
             vx = vx + comp_sd * z.x;
             vy = vy + comp_sd * z.y;
         }
@@ -69,9 +70,10 @@ extern "C" int qsft_assemble_blocks(const qsft_assemble_desc* h) {
 }
 
 extern "C" int qsft_assemble(const qsft_assemble_desc* d, const int32_t* pre, int n_inst, int blocks,
-                             void* stream) {
+                             uint64_t seed_offset, void* stream) {
     if (!d || !pre || n_inst < 1 || blocks < 0) { qsft_set_error("qsft_assemble: bad arguments"); return QSFT_ERR_ARG; }
     if (blocks == 0) return QSFT_OK;
-    hipLaunchKernelGGL(k_assemble, dim3(blocks), dim3(kAsmThreads), 0, (hipStream_t)stream, d, pre, n_inst);
+    hipLaunchKernelGGL(k_assemble, dim3(blocks), dim3(kAsmThreads), 0, (hipStream_t)stream, d, pre, n_inst,
+                       seed_offset);
     return qsft_hip_status(hipGetLastError(), "qsft_assemble");
 }

@@ -379,6 +379,48 @@ __global__ __launch_bounds__(kSlotThreads) void k_peel_finalize(const qsft_peel_
     d.out_order[o] = d.owner[h];
 }
 
+// Exclusive scan of the per-instance output counts into the pack header:
+// hdr[i] = count_i, hdr[n + i] = offset_i, hdr[2n] = total, hdr[2n + 1] = OR of error bits.
+__global__ __launch_bounds__(64) void k_pack_scan(const qsft_peel_desc* __restrict__ descs, int n_inst,
+                                                  int64_t* __restrict__ hdr, int64_t cap_rows) {
+    if (threadIdx.x != 0) return;
+    int64_t off = 0;
+    uint32_t err = 0;
+    for (int i = 0; i < n_inst; ++i) {
+        const qsft_peel_desc& d = descs[i];
+        uint32_t m = d.ctr[QSFT_CTR_OUT];
+        if ((int)m > d.out_cap) m = (uint32_t)d.out_cap;
+        err |= d.ctr[QSFT_CTR_ERROR];
+        hdr[i] = m;
+        hdr[n_inst + i] = off;
+        off += m;
+    }
+    if (off > cap_rows) err |= QSFT_DEVERR_OUT_FULL;
+    hdr[2 * n_inst] = off;
+    hdr[2 * n_inst + 1] = err;
+}
+
+// One block per instance copies its rows: k digits (n_pad bytes), value (2 doubles), order (u64).
+__global__ __launch_bounds__(256) void k_pack_rows(const qsft_peel_desc* __restrict__ descs, int n_inst,
+                                                   const int64_t* __restrict__ hdr, unsigned char* __restrict__ rows,
+                                                   int64_t cap_rows, int row_stride) {
+    const int i = blockIdx.x;
+    const qsft_peel_desc& d = descs[i];
+    const int64_t m = hdr[i], off = hdr[n_inst + i];
+    const int nw = d.n_pad >> 2;
+    for (int64_t r = threadIdx.x; r < m; r += blockDim.x) {
+        if (off + r >= cap_rows) break;
+        unsigned char* dst = rows + (off + r) * row_stride;
+        const uint32_t* ks = reinterpret_cast<const uint32_t*>(d.out_k + r * d.n_pad);
+        uint32_t* kd = reinterpret_cast<uint32_t*>(dst);
+        for (int w = 0; w < nw; ++w) kd[w] = ks[w];
+        double* vd = reinterpret_cast<double*>(dst + d.n_pad);
+        vd[0] = d.out_val[2 * r];
+        vd[1] = d.out_val[2 * r + 1];
+        *reinterpret_cast<uint64_t*>(dst + d.n_pad + 16) = d.out_order[r];
+    }
+}
+
 }  // namespace qsft
 
 using namespace qsft;
@@ -527,4 +569,24 @@ extern "C" int qsft_peel_run(const qsft_peel_desc* d, const int32_t* pre_bins, c
     for (int it = 0; rc == QSFT_OK && it < max_iter; ++it) rc = qsft_peel_iteration(d, pre_bins, n_inst, blocks_bins, it, stream);
     if (rc == QSFT_OK) rc = qsft_peel_finalize(d, pre_slots, n_inst, blocks_slots, stream);
     return rc;
+}
+
+extern "C" size_t qsft_peel_pack_header_bytes(int n_inst) {
+    return n_inst < 1 ? 0 : (((size_t)(2 * n_inst + 2) * 8 + 255) & ~(size_t)255);
+}
+
+extern "C" int qsft_peel_pack(const qsft_peel_desc* d, int n_inst, void* dst, int64_t cap_rows, int row_stride,
+                              void* stream) {
+    if (!d || !dst || n_inst < 1 || cap_rows < 0 || row_stride < 24 + 64 || row_stride % 8) {
+        qsft_set_error("qsft_peel_pack: bad arguments");
+        return QSFT_ERR_ARG;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    int64_t* hdr = static_cast<int64_t*>(dst);
+    unsigned char* rows = static_cast<unsigned char*>(dst) + qsft_peel_pack_header_bytes(n_inst);
+    hipLaunchKernelGGL(k_pack_scan, dim3(1), dim3(64), 0, st, d, n_inst, hdr, cap_rows);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return qsft_hip_status(e, "qsft_peel_pack(scan)");
+    hipLaunchKernelGGL(k_pack_rows, dim3(n_inst), dim3(256), 0, st, d, n_inst, hdr, rows, cap_rows, row_stride);
+    return qsft_hip_status(hipGetLastError(), "qsft_peel_pack(rows)");
 }

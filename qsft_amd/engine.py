@@ -1,0 +1,165 @@
+"""Batched q-SFT engine: many synthetic transforms per launch, state resident in HBM.
+
+A `Batch` uploads its instances once (M_c, D, support digits, strengths),
+allocates samples, sub-transforms, the FP64 working transform and the peel
+workspace per instance, builds every stage's descriptors once, and then
+`launch()` enqueues the whole pipeline with no host work between stages:
+
+    stage (a)+(b)  eval      x[c][p][l] = sum_s a_s w^{<M_c l + d_p, k_s>}   (FP32 or FP64)
+    stage (c)      DFT       in place, x -> q^-b fftn over [q]*b
+    get_MDU step   assemble  FP64 working copy + transform-domain noise (Philox)
+    stage (d)      peel      device loop (no host sync), finalize, pack
+
+`results()` is the only synchronisation: one small header copy and one copy
+of all recovered rows. Uneven instances (different q, n, b, S, C, R) share
+the same launches.
+"""
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from . import _lib as L
+from . import device as dv
+from . import kernels as K
+
+
+@dataclass
+class TransformSpec:
+    """One transform request on a synthetic sparse signal."""
+    q: int
+    n: int
+    b: int
+    C: int
+    R: int
+    locq: np.ndarray                  # (n, S) support digits
+    strengths: np.ndarray             # (S,) complex
+    Ms: list                          # C arrays (n, b)
+    D: np.ndarray                     # (R * (n + 1), n) delay rows shared by all groups
+    noise_sd: float = 0.0
+    method: str = "nso"               # "nso" (nso1) or "identity" (noiseless)
+    seed: int = 0                     # Philox noise seed
+    cutoff: float = None              # default 1e-9 + 1.5 noise_sd^2 / q^b, as the reference
+    tag: object = field(default=None)
+
+    @property
+    def P(self):
+        return self.D.shape[0]
+
+    @property
+    def Q(self):
+        return self.q ** self.b
+
+    @property
+    def S(self):
+        return self.locq.shape[1]
+
+    def threshold(self):
+        if self.cutoff is not None:
+            return self.cutoff
+        return 1e-9 + 1.5 * self.noise_sd ** 2 / self.q ** self.b
+
+    def footprint_bytes(self, precision="fp32"):
+        """HBM bytes this transform holds while resident (samples/transform, FP64
+        working copy, peel workspace, inputs)."""
+        el = 8 if precision == "fp32" else 16
+        cells = self.C * self.P * self.Q
+        ws = L.lib.qsft_peel_workspace_bytes(self.q, self.n, self.b, self.C, self.P, 15, 0)
+        n_pad = dv.n_pad_of(self.n)
+        return cells * (el + 16) + ws + (self.S + self.C * (self.b + self.P)) * n_pad + self.S * el
+
+
+def random_spec(q, n, S, b, C=3, R=2, snr_db=10.0, rng=None, seed=0, distinct=True, method="nso"):
+    """A synthetic request: uniform support (distinct unless told otherwise),
+    |a| ~ U[1, 10], phase ~ U[0, 2 pi), uniform M_c, NSO delays with R random
+    offsets, noise_sd^2 = sum |a|^2 / 10^(snr/10) (None -> noiseless)."""
+    rng = np.random.default_rng(seed) if rng is None else rng
+    locq = rng.integers(0, q, size=(n, S))
+    if distinct:
+        while True:
+            keys = {tuple(c) for c in locq.T}
+            if len(keys) == S:
+                break
+            _, first = np.unique(locq.T, axis=0, return_index=True)
+            dup = np.setdiff1d(np.arange(S), first)
+            locq[:, dup] = rng.integers(0, q, size=(n, len(dup)))
+    a = rng.uniform(1, 10, S) * np.exp(2j * np.pi * rng.uniform(size=S))
+    Ms = [rng.integers(0, q, size=(n, b)) for _ in range(C)]
+    src = np.vstack([np.zeros(n), np.eye(n)]).astype(int)
+    if method == "nso":
+        D = np.vstack([(rng.integers(0, q, n) - src) % q for _ in range(R)])
+    else:
+        R = 1
+        D = src
+    noise_sd = 0.0 if snr_db is None else float(np.sqrt(np.sum(np.abs(a) ** 2) / 10 ** (snr_db / 10)))
+    return TransformSpec(q=q, n=n, b=b, C=C, R=R, locq=locq, strengths=a, Ms=Ms, D=D, noise_sd=noise_sd,
+                         method=method, seed=int(rng.integers(0, 2 ** 62)))
+
+
+class Batch:
+    def __init__(self, specs, precision="fp32", max_iter=15, device=None):
+        self.specs = list(specs)
+        self.precision = precision
+        code, dt = dv.PRECISIONS[precision]
+        dev = device if device is not None else dv.current_device()
+        self.device = dev
+        ev, df, asm, pl = [], [], [], []
+        self.bufs = []
+        for sp in self.specs:
+            n_pad = dv.n_pad_of(sp.n)
+            C, P, Q = sp.C, sp.P, sp.Q
+            Mt = dv.Mt_tensor(sp.Ms, sp.b, n_pad, dev)
+            Dd = dv.D_tensor([sp.D] * C, n_pad, dev)
+            Kd = dv.digits_tensor(sp.locq.T, n_pad, dev)
+            a = dv.strengths_tensor(sp.strengths, precision, dev)
+            x = torch.empty((C, P, Q), dtype=dt, device=dev)
+            U = torch.empty((C, P, Q), dtype=torch.complex128, device=dev)
+            self.bufs.append(dict(Mt=Mt, D=Dd, K=Kd, a=a, x=x, U=U))
+            ev.append(dict(q=sp.q, n=sp.n, b=sp.b, C=C, P=P, S=sp.S, Mt=Mt, D=Dd, K=Kd, a=a, x=x))
+            df.append(dict(q=sp.q, b=sp.b, rows=C * P, inp=x, out=x))
+            item = dict(Q=Q, C=C, R=P // (sp.n + 1), p1=sp.n + 1, src=x, src_rows=P,
+                        gsel=list(range(C)), rsel=list(range(P // (sp.n + 1))), U=U)
+            if sp.noise_sd > 0:
+                item.update(sigma=float(sp.noise_sd / np.sqrt(Q)), seed=sp.seed)
+            asm.append(item)
+            pl.append(dict(q=sp.q, n=sp.n, b=sp.b, C=C, P=P, p1=sp.n + 1,
+                           method=L.QSFT_DETECT_NSO1 if sp.method == "nso" else L.QSFT_DETECT_NOISELESS,
+                           cutoff=sp.threshold(), Mt=Mt, D=Dd, U=U))
+        self.eval = K.EvalCosets(ev, code)
+        self.dft = K.Dft(df, code)
+        self.asm = K.Assemble(asm)
+        self.peel = K.PeelBatch(pl, max_iter=max_iter)
+
+    def launch(self, seed_offset=0, host_loop=False, events=None):
+        """Enqueue the whole pipeline on the current stream. With `events`
+        (a dict), HIP events bracket each stage for timing."""
+        def mark(name):
+            if events is not None:
+                e = torch.cuda.Event(enable_timing=True)
+                e.record()
+                events.setdefault(name, []).append(e)
+        mark("start")
+        self.eval.launch()
+        mark("eval")
+        self.dft.launch()
+        mark("dft")
+        self.asm.launch(seed_offset)
+        mark("assemble")
+        self.peel.run(host_loop=host_loop)
+        mark("peel")
+
+    def results(self, with_info=False):
+        return self.peel.results(with_info=with_info)
+
+
+def nmse(k, vals, spec):
+    """NMSE of recovered (k, vals) against the planted spectrum (the FP64 oracle
+    when q^n is too large for a dense transform)."""
+    truth = {tuple(c): v for c, v in zip(spec.locq.T, spec.strengths)}
+    diff = dict(truth)
+    for row, v in zip(k, vals):
+        t = tuple(row)
+        diff[t] = diff.get(t, 0) - v
+    num = sum(abs(v) ** 2 for v in diff.values())
+    den = sum(abs(v) ** 2 for v in truth.values())
+    return float(num / den)
