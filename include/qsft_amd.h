@@ -49,7 +49,7 @@ enum { QSFT_DEVERR_TABLE_FULL = 1, QSFT_DEVERR_TAG_COLLISION = 2, QSFT_DEVERR_OU
 const char* qsft_last_error(void);
 int qsft_abi_version(void);
 /* sizeof() of the descriptor structs, for FFI layout checks:
- * which = 0 eval, 1 eval_points, 2 dft, 3 assemble, 4 peel. */
+ * which = 0 eval, 1 eval_points, 2 dft, 3 assemble, 4 peel, 5 eval_mfma. */
 size_t qsft_sizeof(int which);
 
 /* ------------------------------------------------------------------------
@@ -80,6 +80,35 @@ typedef struct qsft_eval_desc {
 int qsft_eval_blocks(const qsft_eval_desc* host_desc);
 int qsft_eval_cosets(const qsft_eval_desc* dev_descs, const int32_t* dev_pre, int n_inst,
                      int total_blocks, int precision, void* stream);
+
+/* The same samples on the matrix cores (FP32 result). The phase splits as
+ * <(M_c l + d_p) mod q, k_s> = <(M_c l) mod q, k_s> + <d_p, k_s> (mod q), so
+ * x[c][p][l] = sum_s W_c[l][s] B_c[s][p] with W_c = w^{E0} (E0 from an int8
+ * MFMA over the q^b base queries) and B_c[s][p] = a_s w^{<d_p, k_s>}: a complex
+ * GEMM on bf16 MFMA with B split into three bf16 limbs and FP32 accumulation.
+ * A prep kernel writes B_c into `Bf` (qsft_eval_mfma_workspace_bytes(C, P, S)
+ * bytes) before the main kernel. Replaces the same reference code as
+ * qsft_eval_cosets. */
+typedef struct qsft_eval_mfma_desc {
+    int32_t q, n, b, C, P, S;
+    int32_t n_pad;          /* multiple of 64, >= n                                         */
+    int32_t nsb;            /* (S + 15) / 16  support blocks                                */
+    int32_t npt;            /* (P + 7) / 8    delay-row tiles                               */
+    int32_t shared_d;       /* 1: every group has the same delay rows (B computed once)     */
+    const int8_t* Mt;       /* [C][b][n_pad]                                                */
+    const int8_t* D;        /* [C][P][n_pad]                                                */
+    const int8_t* K;        /* [S][n_pad]                                                   */
+    const double* a;        /* [S][2] strengths, complex FP64                               */
+    uint16_t* Bf;           /* workspace: B fragments, qsft_eval_mfma_workspace_bytes(shared_d ? 1 : C, P, S) */
+    void* x;                /* [C][P][q^b] complex FP32 samples out                         */
+} qsft_eval_mfma_desc;
+
+size_t qsft_eval_mfma_workspace_bytes(int C, int P, int S);
+int qsft_eval_mfma_blocks(const qsft_eval_mfma_desc* host_desc, int* prep_blocks, int* main_blocks);
+/* word: bits 4..7 = n_pad / 64, bits 8..9 = 0 for q in {2, 4}, 1 for other powers of two, 2 otherwise;
+ * one word per launch. */
+int qsft_eval_mfma(const qsft_eval_mfma_desc* dev_descs, const int32_t* dev_pre_prep, const int32_t* dev_pre_main,
+                   int n_inst, int prep_blocks, int main_blocks, int word, void* stream);
 
 /* Stage (a) alone: the digits (M_c l + d_p) mod q of every query, written as
  * int8 [C][P][q^b][n_pad] to desc->x (a, K and S are ignored). Replaces the
@@ -128,8 +157,12 @@ typedef struct qsft_dft_desc {
 } qsft_dft_desc;
 
 int qsft_dft_plan(qsft_dft_desc* host_desc, int precision);
+/* Dynamic LDS bytes one instance's pass needs (tile + twiddles, <= 64 KiB). */
+int qsft_dft_lds_bytes(const qsft_dft_desc* host_desc, int pass, int precision);
+/* lds_bytes: dynamic LDS of the launch, at least qsft_dft_lds_bytes() of every
+ * instance's pass; smaller tiles leave more workgroups resident per CU. */
 int qsft_dft_pass(const qsft_dft_desc* dev_descs, const int32_t* dev_pre, int n_inst,
-                  int total_blocks, int pass, int precision, void* stream);
+                  int total_blocks, int pass, int precision, int lds_bytes, void* stream);
 
 /* ------------------------------------------------------------------------
  * Working transform for one QSFT.transform call: gather the selected groups
@@ -146,14 +179,14 @@ typedef struct qsft_assemble_desc {
     int32_t src_rows;       /* rows per group in the source (= stored repeats * p1)        */
     int32_t src_prec;       /* QSFT_F32 / QSFT_F64                                          */
     int32_t noise_mode;     /* QSFT_NOISE_*                                                 */
-    int32_t _pad0;
+    int32_t bin_major;      /* 1: write U as [C][Q][R*p1] (the peeler's layout)             */
     int32_t gsel[QSFT_MAX_C];   /* source group of working group c                         */
     int32_t rsel[QSFT_MAX_R];   /* source repeat of working repeat r                       */
     const void* src;        /* [C_src][src_rows][Q]                                         */
     const double* noise;    /* QSFT_NOISE_ARRAY: [C][R*p1][Q] complex (re, im) doubles     */
     double sigma;           /* QSFT_NOISE_PHILOX: per-component sd = sqrt(sigma^2 / 2)     */
     uint64_t seed;          /* QSFT_NOISE_PHILOX key                                        */
-    double* U;              /* [C][R*p1][Q] complex FP64 out                                */
+    double* U;              /* [C][R*p1][Q] (or [C][Q][R*p1]) complex FP64 out              */
 } qsft_assemble_desc;
 
 int qsft_assemble_blocks(const qsft_assemble_desc* host_desc);
@@ -166,14 +199,18 @@ int qsft_assemble(const qsft_assemble_desc* dev_descs, const int32_t* dev_pre, i
  * Stage (d): singleton detection + peeling, device resident.
  * Replaces the while-loop of QSFT.transform (qsft/qsft.py:142-255) with
  * singleton_detection (qsft/reconstruct.py:12-31, 100-113, 132-167).
- * Per iteration: kernel A scans every bin of the frozen U (zeroton /
- * singleton / multiton, k, rho, bin match, residual) and registers each
- * singleton in a device hash table keyed by k; kernel B peels each distinct
- * k with the rho of its last (group, bin) from its bin in every group. The
- * stop rule (no singletons or no multitons, the pass still peels), the
- * q^n ball cap and the iteration cap run on the device: no host round trip.
- * The final kernel forms the append-order running mean of every detection
- * of each k (qsft/qsft.py:247-255).
+ * U is bin-major: [C][Q][P] complex FP64, one bin's P rows contiguous.
+ * Per iteration: kernel A (one wave per bin) scans the frozen U -- every bin
+ * on the first pass, afterwards only bins that were non-zeroton on the
+ * previous pass or were peeled into (an unchanged zeroton stays a zeroton) --
+ * classifies zeroton / singleton / multiton (k, rho, bin match, residual) and
+ * registers each singleton in a device hash table keyed by k; kernel B (one
+ * wave per singleton) peels each distinct k with the rho of its last
+ * (group, bin) from its bin in every group. The stop rule (no singletons or
+ * no multitons, the pass still peels), the q^n ball cap and the iteration
+ * cap run on the device: no host round trip. The final kernel forms the
+ * append-order running mean of every detection of each k
+ * (qsft/qsft.py:247-255).
  * ------------------------------------------------------------------------ */
 typedef struct qsft_peel_desc {
     int32_t q, n, b, C, P, Q;
@@ -188,11 +225,15 @@ typedef struct qsft_peel_desc {
     double peel_cap;        /* q^n (as double, saturating): cap on peeled balls            */
     const int8_t* Mt;       /* [C][b][n_pad]                                                */
     const int8_t* D;        /* [C][P][n_pad]                                                */
-    double* U;              /* [C][P][Q] complex FP64, modified in place                   */
+    double* U;              /* [C][Q][P] complex FP64 (bin-major), modified in place       */
     /* workspace, carved by qsft_peel_bind() */
     int8_t* kbin;           /* [C*Q][n_pad]  k of singleton bins of the current pass        */
     double* rho_bin;        /* [C*Q][2]                                                     */
     int32_t* slot_bin;      /* [C*Q]         hash slot of a singleton bin, -1 otherwise     */
+    int32_t* mark;          /* [C*Q]         last pass + 1 whose scan list holds the bin    */
+    int32_t* list0;         /* [C*Q]         scan lists, alternating between passes         */
+    int32_t* list1;         /* [C*Q]                                                        */
+    int32_t* slist;         /* [C*Q]         singleton bins of the current pass             */
     uint64_t* tag;          /* [H] 0 = empty                                                */
     uint64_t* owner;        /* [H] first (iteration, bin) of the key                        */
     uint64_t* last;         /* [H] last (iteration, bin) of the key                         */
@@ -209,14 +250,16 @@ typedef struct qsft_peel_desc {
     uint64_t* out_order;    /* [out_cap]        first (iteration, bin) of k: dict order     */
 } qsft_peel_desc;
 
-/* counter layout (uint32 words) */
-#define QSFT_CTR_ITER_STRIDE 4  /* per iteration: singletons, multitons, balls, active */
-#define QSFT_CTR_LOG 64         /* records appended                                    */
-#define QSFT_CTR_OUT 65         /* outputs written                                     */
-#define QSFT_CTR_ERROR 66       /* QSFT_DEVERR_* bits                                  */
-#define QSFT_CTR_ITERS 67       /* iterations that ran                                 */
-#define QSFT_CTR_PEELED 68      /* balls peeled (saturating)                           */
-#define QSFT_CTR_WORDS 72
+/* counter layout (uint32 words); per iteration i at 8 i + :
+ *   0 singletons, 1 multitons, 2 distinct balls, 3 active flag,
+ *   4 bins in the scan list of pass i + 1, 5 singleton bins of pass i */
+#define QSFT_CTR_ITER_STRIDE 8
+#define QSFT_CTR_LOG 128        /* records appended                                    */
+#define QSFT_CTR_OUT 129        /* outputs written                                     */
+#define QSFT_CTR_ERROR 130      /* QSFT_DEVERR_* bits                                  */
+#define QSFT_CTR_ITERS 131      /* iterations that ran                                 */
+#define QSFT_CTR_PEELED 132     /* balls peeled (saturating)                           */
+#define QSFT_CTR_WORDS 136
 
 size_t qsft_peel_workspace_bytes(int q, int n, int b, int C, int P, int max_iter, int out_cap);
 /* Fill the shape fields and carve `workspace` into the pointer fields. */

@@ -22,9 +22,9 @@ QSFT_NOISE_NONE, QSFT_NOISE_ARRAY, QSFT_NOISE_PHILOX = 0, 1, 2
 QSFT_DEVERR_TABLE_FULL, QSFT_DEVERR_TAG_COLLISION, QSFT_DEVERR_OUT_FULL = 1, 2, 4
 QSFT_MAX_Q, QSFT_MAX_N, QSFT_MAX_B, QSFT_MAX_C, QSFT_MAX_R = 64, 256, 16, 16, 16
 QSFT_DFT_MAX_PASSES = 4
-QSFT_CTR_ITER_STRIDE = 4
-QSFT_CTR_LOG, QSFT_CTR_OUT, QSFT_CTR_ERROR, QSFT_CTR_ITERS, QSFT_CTR_PEELED = 64, 65, 66, 67, 68
-QSFT_CTR_WORDS = 72
+QSFT_CTR_ITER_STRIDE = 8
+QSFT_CTR_LOG, QSFT_CTR_OUT, QSFT_CTR_ERROR, QSFT_CTR_ITERS, QSFT_CTR_PEELED = 128, 129, 130, 131, 132
+QSFT_CTR_WORDS = 136
 
 i32, i64, u64, f64, vp = ctypes.c_int32, ctypes.c_int64, ctypes.c_uint64, ctypes.c_double, ctypes.c_void_p
 
@@ -49,7 +49,7 @@ class DftDesc(ctypes.Structure):
 
 class AssembleDesc(ctypes.Structure):
     _fields_ = [("Q", i32), ("C", i32), ("R", i32), ("p1", i32), ("src_rows", i32), ("src_prec", i32),
-                ("noise_mode", i32), ("_pad0", i32),
+                ("noise_mode", i32), ("bin_major", i32),
                 ("gsel", i32 * QSFT_MAX_C), ("rsel", i32 * QSFT_MAX_R),
                 ("src", vp), ("noise", vp), ("sigma", f64), ("seed", u64), ("U", vp)]
 
@@ -60,13 +60,20 @@ class PeelDesc(ctypes.Structure):
                 ("hcap_log2", i32), ("out_cap", i32), ("_pad0", i32),
                 ("cutoff", f64), ("peel_cap", f64),
                 ("Mt", vp), ("D", vp), ("U", vp),
-                ("kbin", vp), ("rho_bin", vp), ("slot_bin", vp), ("tag", vp), ("owner", vp),
+                ("kbin", vp), ("rho_bin", vp), ("slot_bin", vp), ("mark", vp), ("list0", vp), ("list1", vp),
+                ("slist", vp), ("tag", vp), ("owner", vp),
                 ("last", vp), ("head", vp), ("count", vp), ("key", vp), ("rec_time", vp),
                 ("rec_rho", vp), ("rec_next", vp), ("ctr", vp),
                 ("out_k", vp), ("out_val", vp), ("out_order", vp)]
 
 
-STRUCTS = {0: EvalDesc, 1: EvalPointsDesc, 2: DftDesc, 3: AssembleDesc, 4: PeelDesc}
+class EvalMfmaDesc(ctypes.Structure):
+    _fields_ = [("q", i32), ("n", i32), ("b", i32), ("C", i32), ("P", i32), ("S", i32),
+                ("n_pad", i32), ("nsb", i32), ("npt", i32), ("shared_d", i32),
+                ("Mt", vp), ("D", vp), ("K", vp), ("a", vp), ("Bf", vp), ("x", vp)]
+
+
+STRUCTS = {0: EvalDesc, 1: EvalPointsDesc, 2: DftDesc, 3: AssembleDesc, 4: PeelDesc, 5: EvalMfmaDesc}
 
 # exported symbols (every function declared in include/qsft_amd.h)
 SIGNATURES = {
@@ -75,12 +82,18 @@ SIGNATURES = {
     "qsft_sizeof": (ctypes.c_size_t, [ctypes.c_int]),
     "qsft_eval_blocks": (ctypes.c_int, [ctypes.POINTER(EvalDesc)]),
     "qsft_eval_cosets": (ctypes.c_int, [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp]),
+    "qsft_eval_mfma_workspace_bytes": (ctypes.c_size_t, [ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    "qsft_eval_mfma_blocks": (ctypes.c_int, [ctypes.POINTER(EvalMfmaDesc), ctypes.POINTER(ctypes.c_int),
+                                             ctypes.POINTER(ctypes.c_int)]),
+    "qsft_eval_mfma": (ctypes.c_int, [vp, vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp]),
     "qsft_query_digits_blocks": (ctypes.c_int, [ctypes.POINTER(EvalDesc)]),
     "qsft_query_digits": (ctypes.c_int, [vp, vp, ctypes.c_int, ctypes.c_int, vp]),
     "qsft_eval_points_blocks": (ctypes.c_int, [ctypes.POINTER(EvalPointsDesc)]),
     "qsft_eval_points": (ctypes.c_int, [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp]),
     "qsft_dft_plan": (ctypes.c_int, [ctypes.POINTER(DftDesc), ctypes.c_int]),
-    "qsft_dft_pass": (ctypes.c_int, [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp]),
+    "qsft_dft_lds_bytes": (ctypes.c_int, [ctypes.POINTER(DftDesc), ctypes.c_int, ctypes.c_int]),
+    "qsft_dft_pass": (ctypes.c_int, [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                     vp]),
     "qsft_assemble_blocks": (ctypes.c_int, [ctypes.POINTER(AssembleDesc)]),
     "qsft_assemble": (ctypes.c_int, [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_uint64, vp]),
     "qsft_peel_workspace_bytes": (ctypes.c_size_t, [ctypes.c_int] * 7),

@@ -31,6 +31,7 @@ extern "C" size_t qsft_sizeof(int which) {
         case 2: return sizeof(qsft_dft_desc);
         case 3: return sizeof(qsft_assemble_desc);
         case 4: return sizeof(qsft_peel_desc);
+        case 5: return sizeof(qsft_eval_mfma_desc);
         default: return 0;
     }
 }

@@ -87,11 +87,10 @@ __device__ __forceinline__ void pencil_generic_out(const cplx_t<T>* src, cplx_t<
 template <typename T, int QR>
 __global__ __launch_bounds__(kDftThreads) void k_dft_pass(const qsft_dft_desc* __restrict__ descs,
                                                           const int32_t* __restrict__ pre, int n_inst,
-                                                          int pass) {
+                                                          int pass, int lds_bytes) {
     using C2 = cplx_t<T>;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     C2* lds = reinterpret_cast<C2*>(smem_raw);
-    C2* tw = reinterpret_cast<C2*>(smem_raw + kDftTileBytes);
 
     const int inst = find_inst(pre, n_inst, blockIdx.x);
     const qsft_dft_desc& d = descs[inst];
@@ -111,6 +110,9 @@ __global__ __launch_bounds__(kDftThreads) void k_dft_pass(const qsft_dft_desc* _
     const int i0 = (rt - a_idx * ichunks) * ti;
     const int64_t base = (int64_t)a_idx * M * I + i0;       // element offset inside the row
     const int E = M * ti;
+    // twiddles follow the tile (two tiles for the generic radix's ping-pong)
+    C2* tw = lds + (QR > 0 ? E : 2 * E);
+    if ((int)(((QR > 0 ? E : 2 * E) + q) * sizeof(C2)) > lds_bytes) return;    // launch sized too small
 
     for (int r = threadIdx.x; r < q; r += blockDim.x) {
         double s, c;
@@ -238,24 +240,35 @@ extern "C" int qsft_dft_plan(qsft_dft_desc* h, int precision) {
 
 template <typename T>
 static hipError_t launch_dft(int radix, const qsft_dft_desc* d, const int32_t* pre, int n_inst, int blocks,
-                             int pass, hipStream_t st) {
-    const size_t sh = kDftLdsBytes;
+                             int pass, int lds, hipStream_t st) {
+    const size_t sh = (size_t)lds;
     switch (radix) {
-        case 2: hipLaunchKernelGGL((k_dft_pass<T, 2>), dim3(blocks), dim3(kDftThreads), sh, st, d, pre, n_inst, pass); break;
-        case 3: hipLaunchKernelGGL((k_dft_pass<T, 3>), dim3(blocks), dim3(kDftThreads), sh, st, d, pre, n_inst, pass); break;
-        case 4: hipLaunchKernelGGL((k_dft_pass<T, 4>), dim3(blocks), dim3(kDftThreads), sh, st, d, pre, n_inst, pass); break;
-        case 5: hipLaunchKernelGGL((k_dft_pass<T, 5>), dim3(blocks), dim3(kDftThreads), sh, st, d, pre, n_inst, pass); break;
-        case 20: hipLaunchKernelGGL((k_dft_pass<T, 20>), dim3(blocks), dim3(kDftThreads), sh, st, d, pre, n_inst, pass); break;
-        default: hipLaunchKernelGGL((k_dft_pass<T, 0>), dim3(blocks), dim3(kDftThreads), sh, st, d, pre, n_inst, pass); break;
+        case 2: hipLaunchKernelGGL((k_dft_pass<T, 2>), dim3(blocks), dim3(kDftThreads), sh, st, d, pre, n_inst, pass, lds); break;
+        case 3: hipLaunchKernelGGL((k_dft_pass<T, 3>), dim3(blocks), dim3(kDftThreads), sh, st, d, pre, n_inst, pass, lds); break;
+        case 4: hipLaunchKernelGGL((k_dft_pass<T, 4>), dim3(blocks), dim3(kDftThreads), sh, st, d, pre, n_inst, pass, lds); break;
+        case 5: hipLaunchKernelGGL((k_dft_pass<T, 5>), dim3(blocks), dim3(kDftThreads), sh, st, d, pre, n_inst, pass, lds); break;
+        case 20: hipLaunchKernelGGL((k_dft_pass<T, 20>), dim3(blocks), dim3(kDftThreads), sh, st, d, pre, n_inst, pass, lds); break;
+        default: hipLaunchKernelGGL((k_dft_pass<T, 0>), dim3(blocks), dim3(kDftThreads), sh, st, d, pre, n_inst, pass, lds); break;
     }
     return hipGetLastError();
 }
 
+extern "C" int qsft_dft_lds_bytes(const qsft_dft_desc* h, int pass, int precision) {
+    if (!h || pass < 0 || pass >= h->n_pass || (precision != QSFT_F32 && precision != QSFT_F64)) return -1;
+    const long long elem = (precision == QSFT_F32) ? 8 : 16;
+    const long long E = (long long)ipow(h->q, h->pass_d1[pass] - h->pass_d0[pass]) * h->pass_ti[pass];
+    const long long bytes = (radix_compiled(h->q) ? E : 2 * E) * elem + (long long)h->q * elem;
+    return bytes > kDftLdsBytes ? -1 : (int)bytes;
+}
+
 // All instances of one launch share q (radix kernels are compiled per q);
-// `precision` low nibble = QSFT_F32/F64, bits 8..15 = q.
+// `precision` low nibble = QSFT_F32/F64, bits 8..15 = q. lds_bytes >= the
+// qsft_dft_lds_bytes() of every instance's pass (a block whose tile does not
+// fit does nothing).
 extern "C" int qsft_dft_pass(const qsft_dft_desc* d, const int32_t* pre, int n_inst, int blocks, int pass,
-                             int precision, void* stream) {
-    if (!d || !pre || n_inst < 1 || blocks < 0 || pass < 0 || pass >= QSFT_DFT_MAX_PASSES) {
+                             int precision, int lds_bytes, void* stream) {
+    if (!d || !pre || n_inst < 1 || blocks < 0 || pass < 0 || pass >= QSFT_DFT_MAX_PASSES || lds_bytes < 16 ||
+        lds_bytes > kDftLdsBytes) {
         qsft_set_error("qsft_dft_pass: bad arguments");
         return QSFT_ERR_ARG;
     }
@@ -268,7 +281,7 @@ extern "C" int qsft_dft_pass(const qsft_dft_desc* d, const int32_t* pre, int n_i
     }
     const int radix = radix_compiled(q) ? q : 0;
     hipStream_t st = (hipStream_t)stream;
-    const hipError_t e = (prec == QSFT_F32) ? launch_dft<float>(radix, d, pre, n_inst, blocks, pass, st)
-                                            : launch_dft<double>(radix, d, pre, n_inst, blocks, pass, st);
+    const hipError_t e = (prec == QSFT_F32) ? launch_dft<float>(radix, d, pre, n_inst, blocks, pass, lds_bytes, st)
+                                            : launch_dft<double>(radix, d, pre, n_inst, blocks, pass, lds_bytes, st);
     return qsft_hip_status(e, "qsft_dft_pass");
 }

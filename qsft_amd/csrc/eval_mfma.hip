@@ -1,0 +1,429 @@
+// Stage (a)+(b) on the matrix cores: coset evaluation as a complex GEMM.
+//
+// Every query of group c is m = (M_c l + d_p) mod q, so its phase against a
+// support vector splits exactly (mod q is a ring homomorphism):
+//   <m, k_s> = <(M_c l) mod q, k_s> + <d_p, k_s>   (mod q)
+// and the sample is
+//   x[c][p][l] = sum_s W_c[l][s] B_c[s][p],
+//   W_c[l][s] = w^{E0},  E0 = <(M_c l) mod q, k_s> mod q,
+//   B_c[s][p] = a_s w^{<d_p, k_s> mod q}.
+// Each sample is still the full S-term sum of a_s w^{<m, k_s>}; the B*S
+// complex multiply-adds run as a (Q x S) x (S x P) complex GEMM on
+// v_mfma_f32_16x16x32_bf16, and the w-table epilogue runs C*Q*S times
+// instead of C*P*Q*S times.
+//
+//   * E0 comes from v_mfma_i32_16x16x64_i8 (support digits x base-query digits,
+//     exact int32); its D fragment (lane = query column l, 4 rows = supports
+//     4g..4g+3) is exactly the bf16 A fragment needed next: slots 0..3 hold
+//     Re W for those 4 supports and slots 4..7 Im W.
+//   * B is precomputed (prep kernel) in FP64, split into three bf16 limbs
+//     (hi + mid + lo carries ~24 significant bits), and stored already in the
+//     B-fragment order of the bf16 MFMA, with the complex product laid out as
+//     a real GEMM: column 2p' holds Re x, column 2p'+1 holds Im x, and the
+//     "Im W" k-slots carry -Im B / Re B. When all groups share their delay
+//     rows (the reference's get_Ms_and_Ds), B is computed once.
+//   * w^e is exact in bf16 for q in {2, 4} (0, +-1); other q split W into
+//     three limbs too and keep the six limb products of order <= 2.
+//   * accumulation is FP32 in the MFMA; x is written as complex FP32.
+//
+// A workgroup owns 256 base queries (4 waves x 4 subtiles of 16) x 8 p-tiles
+// (64 delay rows) of one group. Per 16-support block it stages the 8 x 3 B
+// fragments (24 KiB) in LDS and each wave reuses every fragment for its 4
+// subtiles; the next block's fragments are loaded into registers while the
+// current block's MFMAs run. Logical blocks are dealt out XCD by XCD so the
+// workgroups that share B fragments share an L2.
+#include "qsft_common.h"
+
+namespace qsft {
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef unsigned int v4u __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+
+constexpr int kMfThreads = 256;
+constexpr int kRowsPerBlock = 256;   // base queries per workgroup
+constexpr int kPtPerBlock = 8;       // p-tiles (8 complex delay rows each) per workgroup
+constexpr int kLimbs = 3;            // bf16 limbs of B
+constexpr int kStage = kPtPerBlock * kLimbs * 64;      // v4u per staged support block
+constexpr int kStagePerThread = kStage / kMfThreads;   // 6
+constexpr int kXcds = 8;
+
+// round-to-nearest-even float -> bf16 bits
+__host__ __device__ __forceinline__ uint32_t bf16_bits(float f) {
+    uint32_t u = __builtin_bit_cast(uint32_t, f);
+    if ((u & 0x7F800000u) == 0x7F800000u) return u >> 16;        // inf / nan (not produced here)
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return u >> 16;
+}
+__host__ __device__ __forceinline__ double bf16_to_double(uint32_t b) {
+    return (double)__builtin_bit_cast(float, b << 16);
+}
+// v = limb0 + limb1 + limb2 (+ O(2^-27 |v|))
+__host__ __device__ __forceinline__ void split3(double v, uint32_t (&l)[3]) {
+    double r = v;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        l[i] = bf16_bits((float)r);
+        r -= bf16_to_double(l[i]);
+    }
+}
+
+__device__ __forceinline__ v4i pack16b(const int (&d)[16]) {
+    v4i r;
+#pragma unroll
+    for (int w = 0; w < 4; ++w)
+        r[w] = (d[4 * w] & 0xFF) | ((d[4 * w + 1] & 0xFF) << 8) | ((d[4 * w + 2] & 0xFF) << 16) |
+               ((d[4 * w + 3] & 0xFF) << 24);
+    return r;
+}
+
+__device__ __forceinline__ int byte_at(const v4i& v, int j) { return (v[j >> 2] >> (8 * (j & 3))) & 0xFF; }
+
+// Physical workgroup -> logical workgroup so that each XCD (workgroups are
+// dealt round-robin to the 8 XCDs) runs a contiguous range of logical ids.
+__device__ __forceinline__ int xcd_logical_block() {
+    const int nb = gridDim.x, bid = blockIdx.x;
+    const int xcd = bid % kXcds, slot = bid / kXcds;
+    const int per = nb / kXcds, rem = nb % kXcds;
+    return (xcd < rem) ? xcd * (per + 1) + slot : rem * (per + 1) + (xcd - rem) * per + slot;
+}
+
+// Prep: B_c[s][p] = a_s w^{<d_cp, k_s>} in bf16-limb B-fragment order.
+// Fragment (c, sb, pt, limb) is 64 lanes x 8 bf16; lane = 16 g + n with
+// n = 2 (p % 8) + {0: Re column, 1: Im column}; slot j < 4 <-> "Re W" row of
+// support s = 16 sb + 4 g + j, slot 4 + j <-> "Im W" row of the same support.
+// One thread fills lanes (16 g + 2 pl, 16 g + 2 pl + 1) of every limb for one
+// (c, sb, pt, g, pl): two adjacent 16-byte lanes per limb.
+__global__ __launch_bounds__(256) void k_eval_mfma_prep(const qsft_eval_mfma_desc* __restrict__ descs,
+                                                        const int32_t* __restrict__ pre, int n_inst) {
+    __shared__ double2 tab[QSFT_MAX_Q];
+    const int inst = find_inst(pre, n_inst, blockIdx.x);
+    const qsft_eval_mfma_desc& d = descs[inst];
+    const int q = d.q, n_pad = d.n_pad, nsb = d.nsb, npt = d.npt;
+    for (int r = threadIdx.x; r < q; r += blockDim.x) {
+        double s, c;
+        sincospi(2.0 * (double)r / (double)q, &s, &c);
+        tab[r] = make_double2(c, s);
+    }
+    __syncthreads();
+    const int Cb = d.shared_d ? 1 : d.C;
+    const int64_t idx = (int64_t)(blockIdx.x - pre[inst]) * blockDim.x + threadIdx.x;
+    const int64_t per_c = (int64_t)nsb * npt * 32;          // (sb, pt, g, pl)
+    if (idx >= per_c * Cb) return;
+    const int c = (int)(idx / per_c);
+    int64_t rem = idx - c * per_c;
+    const int pl = (int)(rem & 7);
+    rem >>= 3;
+    const int g = (int)(rem & 3);
+    rem >>= 2;
+    const int pt = (int)(rem % npt);
+    const int sb = (int)(rem / npt);
+    const int p = 8 * pt + pl;
+    const ModQ mq((uint32_t)q);
+    uint32_t lr[4][3], li[4][3];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int s = 16 * sb + 4 * g + j;
+        double br = 0.0, bi = 0.0;
+        if (s < d.S && p < d.P) {
+            const int* __restrict__ dr = reinterpret_cast<const int*>(d.D + ((size_t)c * d.P + p) * n_pad);
+            const int* __restrict__ kr = reinterpret_cast<const int*>(d.K + (size_t)s * n_pad);
+            int acc = 0;
+            for (int w = 0; w < (n_pad >> 2); ++w) acc = __builtin_amdgcn_sdot4(dr[w], kr[w], acc, false);
+            const double2 wv = tab[mq((uint32_t)acc)];
+            const double ar = d.a[2 * s], ai = d.a[2 * s + 1];
+            br = ar * wv.x - ai * wv.y;
+            bi = ar * wv.y + ai * wv.x;
+        }
+        split3(br, lr[j]);
+        split3(bi, li[j]);
+    }
+    v4u* __restrict__ out = reinterpret_cast<v4u*>(d.Bf);
+#pragma unroll
+    for (int limb = 0; limb < kLimbs; ++limb) {
+        const size_t frag = ((((size_t)c * nsb + sb) * npt + pt) * kLimbs + limb) * 64;
+        // Re x column: Re W rows carry Br, Im W rows carry -Bi
+        const v4u re = v4u{lr[0][limb] | (lr[1][limb] << 16), lr[2][limb] | (lr[3][limb] << 16),
+                           (li[0][limb] ^ 0x8000u) | ((li[1][limb] ^ 0x8000u) << 16),
+                           (li[2][limb] ^ 0x8000u) | ((li[3][limb] ^ 0x8000u) << 16)};
+        // Im x column: Re W rows carry Bi, Im W rows carry Br
+        const v4u im = v4u{li[0][limb] | (li[1][limb] << 16), li[2][limb] | (li[3][limb] << 16),
+                           lr[0][limb] | (lr[1][limb] << 16), lr[2][limb] | (lr[3][limb] << 16)};
+        out[frag + 16 * g + 2 * pl] = re;
+        out[frag + 16 * g + 2 * pl + 1] = im;
+    }
+}
+
+// MODE 0: q in {2, 4} (w^e exact in bf16, q a power of two)
+// MODE 1: other powers of two (3-limb W)   MODE 2: any other q (3-limb W)
+template <int MODE, int NKB>
+__global__ __launch_bounds__(kMfThreads) void k_eval_mfma(const qsft_eval_mfma_desc* __restrict__ descs,
+                                                          const int32_t* __restrict__ pre, int n_inst) {
+    constexpr int WL = (MODE == 0) ? 1 : 3;
+    __shared__ __align__(16) v4u bsh[kStage];                     // 24 KiB of B fragments
+    __shared__ uint32_t wtab[3][QSFT_MAX_Q];                      // (Re | Im << 16) bf16 bits per limb
+
+    const int lblock = xcd_logical_block();
+    const int inst = find_inst(pre, n_inst, lblock);
+    const qsft_eval_mfma_desc& d = descs[inst];
+    const int q = d.q, b = d.b, n_pad = d.n_pad, nsb = d.nsb, npt = d.npt, P = d.P, S = d.S;
+    const int Q = ipow(q, b);
+    const int nlb = (Q + kRowsPerBlock - 1) / kRowsPerBlock;
+    const int ncg = (npt + kPtPerBlock - 1) / kPtPerBlock;
+    int local = lblock - pre[inst];
+    const int cg = local % ncg;
+    local /= ncg;
+    const int lb = local % nlb;
+    const int c = local / nlb;
+    const int cb = d.shared_d ? 0 : c;                            // group whose B fragments apply
+    const int pt0 = cg * kPtPerBlock;
+    const int npt_here = min(kPtPerBlock, npt - pt0);
+    const int nstage = npt_here * kLimbs * 64;
+
+    for (int r = threadIdx.x; r < q; r += blockDim.x) {
+        double s, cc;
+        sincospi(2.0 * (double)r / (double)q, &s, &cc);
+        uint32_t lc[3], ls[3];
+        split3(cc, lc);
+        split3(s, ls);
+        for (int i = 0; i < 3; ++i) wtab[i][r] = lc[i] | (ls[i] << 16);
+    }
+
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int g = lane >> 4, r16 = lane & 15;
+    const ModQ mq((uint32_t)q);
+    const int lbase = lb * kRowsPerBlock + wave * 64;
+
+    // base-query digit fragments (B operand of the int8 MFMA): column l = lbase + 16 t + r16
+    v4i bq[4][NKB];
+    const int8_t* __restrict__ Mt = d.Mt + (size_t)c * b * n_pad;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        const int l = lbase + 16 * t + r16;
+#pragma unroll
+        for (int kb = 0; kb < NKB; ++kb) {
+            const int i0 = 64 * kb + 16 * g;
+            int acc[16];
+#pragma unroll
+            for (int jj = 0; jj < 16; ++jj) acc[jj] = 0;
+            int rem = l < Q ? l : 0, pw = Q;
+            for (int tt = 0; tt < b; ++tt) {
+                pw /= q;
+                const int lt = rem / pw;
+                rem -= lt * pw;
+                const v4i mv = *reinterpret_cast<const v4i*>(Mt + (size_t)tt * n_pad + i0);
+#pragma unroll
+                for (int jj = 0; jj < 16; ++jj) acc[jj] += lt * byte_at(mv, jj);
+            }
+#pragma unroll
+            for (int jj = 0; jj < 16; ++jj) acc[jj] = (int)mq((uint32_t)acc[jj]);
+            bq[t][kb] = pack16b(acc);
+        }
+    }
+
+    v4f acc[4][kPtPerBlock];
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int pt = 0; pt < kPtPerBlock; ++pt) acc[t][pt] = v4f{0.f, 0.f, 0.f, 0.f};
+
+    const v4u* __restrict__ Bg = reinterpret_cast<const v4u*>(d.Bf);
+    const int8_t* __restrict__ Kd = d.K;
+    auto frag_base = [&](int sb) { return (((size_t)cb * nsb + sb) * npt + pt0) * kLimbs * 64; };
+    auto load_k = [&](int sb, v4i (&kf)[NKB]) {
+        const int s = 16 * sb + r16;
+#pragma unroll
+        for (int kb = 0; kb < NKB; ++kb) {
+            v4i z = {0, 0, 0, 0};
+            if (s < S) z = *reinterpret_cast<const v4i*>(Kd + (size_t)s * n_pad + 64 * kb + 16 * g);
+            kf[kb] = z;
+        }
+    };
+
+    // prologue: support block 0 into registers, then into LDS
+    v4u stage[kStagePerThread];
+    v4i kf[NKB];
+    {
+        const size_t fb = frag_base(0);
+#pragma unroll
+        for (int i = 0; i < kStagePerThread; ++i) {
+            const int e = tid + i * kMfThreads;
+            if (e < nstage) stage[i] = Bg[fb + e];
+        }
+        load_k(0, kf);
+#pragma unroll
+        for (int i = 0; i < kStagePerThread; ++i) {
+            const int e = tid + i * kMfThreads;
+            if (e < nstage) bsh[e] = stage[i];
+        }
+    }
+    __syncthreads();     // fragments of block 0 and wtab visible
+
+    for (int sb = 0; sb < nsb; ++sb) {
+        // W fragments for the 4 subtiles of this support block
+        v4u wa[4][WL];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            v4i e = {0, 0, 0, 0};
+#pragma unroll
+            for (int kb = 0; kb < NKB; ++kb) e = __builtin_amdgcn_mfma_i32_16x16x64_i8(kf[kb], bq[t][kb], e, 0, 0, 0);
+            uint32_t r[4];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) r[v] = (MODE <= 1) ? ((uint32_t)e[v] & (uint32_t)(q - 1)) : mq((uint32_t)e[v]);
+            if constexpr (MODE == 0) {
+                // q in {2, 4}: Re w^e = {1, 0, -1, 0}, Im w^e = {0, 1, 0, -1}; for q = 2, e in {0, 1} -> {1, -1}
+                uint32_t wr[4], wi[4];
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const uint32_t ee = (q == 2) ? 2 * r[v] : r[v];
+                    wr[v] = (ee == 0) ? 0x3F80u : (ee == 2 ? 0xBF80u : 0u);
+                    wi[v] = (ee == 1) ? 0x3F80u : (ee == 3 ? 0xBF80u : 0u);
+                }
+                wa[t][0] = v4u{wr[0] | (wr[1] << 16), wr[2] | (wr[3] << 16), wi[0] | (wi[1] << 16), wi[2] | (wi[3] << 16)};
+            } else {
+#pragma unroll
+                for (int li = 0; li < WL; ++li) {
+                    uint32_t w[4];
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) w[v] = wtab[li][r[v]];
+                    wa[t][li] = v4u{(w[0] & 0xFFFFu) | (w[1] << 16), (w[2] & 0xFFFFu) | (w[3] << 16),
+                                    (w[0] >> 16) | (w[1] & 0xFFFF0000u), (w[2] >> 16) | (w[3] & 0xFFFF0000u)};
+                }
+            }
+        }
+        // prefetch the next support block while the MFMAs below run
+        const bool more = sb + 1 < nsb;
+        if (more) {
+            const size_t fb = frag_base(sb + 1);
+#pragma unroll
+            for (int i = 0; i < kStagePerThread; ++i) {
+                const int e = tid + i * kMfThreads;
+                if (e < nstage) stage[i] = Bg[fb + e];
+            }
+            load_k(sb + 1, kf);
+        }
+#pragma unroll
+        for (int pt = 0; pt < kPtPerBlock; ++pt) {
+            if (pt < npt_here) {
+#pragma unroll
+                for (int bl = 0; bl < kLimbs; ++bl) {
+                    const v8bf bf = __builtin_bit_cast(v8bf, bsh[(pt * kLimbs + bl) * 64 + lane]);
+#pragma unroll
+                    for (int wl = 0; wl < WL; ++wl) {
+                        if (wl + bl > 2) continue;          // drop limb products below 2^-24
+#pragma unroll
+                        for (int t = 0; t < 4; ++t)
+                            acc[t][pt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(v8bf, wa[t][wl]), bf,
+                                                                               acc[t][pt], 0, 0, 0);
+                    }
+                }
+            }
+        }
+        __syncthreads();     // every wave is done with this block's fragments
+        if (more) {
+#pragma unroll
+            for (int i = 0; i < kStagePerThread; ++i) {
+                const int e = tid + i * kMfThreads;
+                if (e < nstage) bsh[e] = stage[i];
+            }
+        }
+        __syncthreads();     // next block's fragments visible
+    }
+
+    // store: lane holds column n = r16 -> delay row p = 8 (pt0 + pt) + n / 2, Re (n even) / Im (n odd),
+    // for base queries l = lbase + 16 t + 4 g + v. Pair Re/Im across lanes (xor 1).
+    float2* __restrict__ x = reinterpret_cast<float2*>(d.x) + (size_t)c * P * Q;
+    const bool odd = r16 & 1;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        const int l0 = lbase + 16 * t + 4 * g;
+#pragma unroll
+        for (int pt = 0; pt < kPtPerBlock; ++pt) {
+            if (pt < npt_here) {
+                const v4f a = acc[t][pt];
+                v4f o;
+#pragma unroll
+                for (int v = 0; v < 4; ++v) o[v] = __shfl_xor(a[v], 1);
+                const int p = 8 * (pt0 + pt) + (r16 >> 1);
+                if (p < P) {
+                    float2* row = x + (size_t)p * Q;
+                    if (!odd) {
+                        if (l0 < Q) row[l0] = make_float2(a[0], o[0]);
+                        if (l0 + 1 < Q) row[l0 + 1] = make_float2(a[1], o[1]);
+                    } else {
+                        if (l0 + 2 < Q) row[l0 + 2] = make_float2(o[2], a[2]);
+                        if (l0 + 3 < Q) row[l0 + 3] = make_float2(o[3], a[3]);
+                    }
+                }
+            }
+        }
+    }
+}
+
+}  // namespace qsft
+
+using namespace qsft;
+
+extern "C" void qsft_set_error(const char* msg);
+extern "C" int qsft_hip_status(hipError_t e, const char* where);
+
+static bool mfma_desc_ok(const qsft_eval_mfma_desc* h) {
+    return h && h->q >= 2 && h->q <= QSFT_MAX_Q && h->b >= 1 && h->b <= QSFT_MAX_B && h->n >= 1 && h->n <= h->n_pad &&
+           h->n_pad % 64 == 0 && h->n_pad <= QSFT_MAX_N && h->C >= 1 && h->P >= 1 && h->S >= 0 &&
+           h->nsb == (h->S + 15) / 16 && h->npt == (h->P + 7) / 8 && (h->shared_d == 0 || h->shared_d == 1);
+}
+
+extern "C" size_t qsft_eval_mfma_workspace_bytes(int C, int P, int S) {
+    if (C < 1 || P < 1 || S < 0) return 0;
+    return (size_t)C * ((S + 15) / 16) * ((P + 7) / 8) * kLimbs * 1024;
+}
+
+extern "C" int qsft_eval_mfma_blocks(const qsft_eval_mfma_desc* h, int* prep_blocks, int* main_blocks) {
+    if (!mfma_desc_ok(h) || !prep_blocks || !main_blocks) return -1;
+    const long long Q = ipow(h->q, h->b);
+    const long long Cb = h->shared_d ? 1 : h->C;
+    const long long prep = (Cb * h->nsb * h->npt * 32 + 255) / 256;
+    const long long mainb = (long long)h->C * ((Q + kRowsPerBlock - 1) / kRowsPerBlock) *
+                            ((h->npt + kPtPerBlock - 1) / kPtPerBlock);
+    if (prep > 0x7FFFFFFF || mainb > 0x7FFFFFFF) return -1;
+    *prep_blocks = (int)prep;
+    *main_blocks = (int)mainb;
+    return 0;
+}
+
+template <int MODE>
+static hipError_t launch_mfma(int nkb, const qsft_eval_mfma_desc* d, const int32_t* pre, int n, int blocks,
+                              hipStream_t st) {
+    switch (nkb) {
+        case 1: hipLaunchKernelGGL((k_eval_mfma<MODE, 1>), dim3(blocks), dim3(kMfThreads), 0, st, d, pre, n); break;
+        case 2: hipLaunchKernelGGL((k_eval_mfma<MODE, 2>), dim3(blocks), dim3(kMfThreads), 0, st, d, pre, n); break;
+        case 3: hipLaunchKernelGGL((k_eval_mfma<MODE, 3>), dim3(blocks), dim3(kMfThreads), 0, st, d, pre, n); break;
+        default: hipLaunchKernelGGL((k_eval_mfma<MODE, 4>), dim3(blocks), dim3(kMfThreads), 0, st, d, pre, n); break;
+    }
+    return hipGetLastError();
+}
+
+// word: bits 4..7 digit-block count (1..4), bits 8..9 mode (0: q in {2,4}; 1: other powers of two; 2: other q).
+// All instances of one launch share the word.
+extern "C" int qsft_eval_mfma(const qsft_eval_mfma_desc* d, const int32_t* pre_prep, const int32_t* pre_main,
+                              int n_inst, int prep_blocks, int main_blocks, int word, void* stream) {
+    if (!d || !pre_prep || !pre_main || n_inst < 1 || prep_blocks < 0 || main_blocks < 0) {
+        qsft_set_error("qsft_eval_mfma: bad arguments");
+        return QSFT_ERR_ARG;
+    }
+    const int nkb = (word >> 4) & 0xF, mode = (word >> 8) & 0x3;
+    if (nkb < 1 || nkb > 4 || mode > 2) { qsft_set_error("qsft_eval_mfma: bad word"); return QSFT_ERR_ARG; }
+    hipStream_t st = (hipStream_t)stream;
+    if (prep_blocks > 0) {
+        hipLaunchKernelGGL(k_eval_mfma_prep, dim3(prep_blocks), dim3(256), 0, st, d, pre_prep, n_inst);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return qsft_hip_status(e, "qsft_eval_mfma(prep)");
+    }
+    if (main_blocks == 0) return QSFT_OK;
+    hipError_t e;
+    if (mode == 0) e = launch_mfma<0>(nkb, d, pre_main, n_inst, main_blocks, st);
+    else if (mode == 1) e = launch_mfma<1>(nkb, d, pre_main, n_inst, main_blocks, st);
+    else e = launch_mfma<2>(nkb, d, pre_main, n_inst, main_blocks, st);
+    return qsft_hip_status(e, "qsft_eval_mfma");
+}

@@ -2,7 +2,7 @@
 //
 // Semantics follow QSFT.transform (qsft/qsft.py:142-255) and
 // singleton_detection (qsft/reconstruct.py:12-31, 100-113):
-//   * every pass scans all bins (c, j) of a frozen U (Jacobi): zeroton if
+//   * every pass classifies the bins (c, j) of a frozen U (Jacobi): zeroton if
 //     |col|^2 <= cutoff * P; otherwise k from nso1 or the noiseless rule,
 //     signature w^{D k}, rho = sig^H col / P, residual col - rho sig; a
 //     singleton iff M_c^T k mod q equals the digits of j and the residual
@@ -15,25 +15,47 @@
 //   * the output value of k is the running mean, in detection order, of
 //     every rho ever detected for it.
 //
-// Layout: U is [C][P][Q] complex FP64 (row-major, the DFT's output); one
-// thread owns one bin and walks its P-column with stride Q, so a wave reads
-// 64 consecutive bins of a row per access (1 KiB, coalesced).
+// Layout: U is bin-major [C][Q][P] complex FP64, so one bin's column is P
+// contiguous 16-byte rows. One wave owns one bin at a time: lanes hold rows
+// p = lane + 64 r (P <= 256) for the energy, rho and residual sums (FP64
+// xor-butterfly reductions, bit-identical on every lane) and coordinates
+// i = lane + 1 (+ 64) for nso1 (one atan2 per lane).
+//
+// Scan lists: pass 0 scans every bin; a later pass scans only the bins that
+// were non-zeroton on the previous pass or were peeled into by it (an
+// untouched zeroton keeps its column, so it stays a zeroton). A per-bin
+// stamp (atomicMax with pass + 1) keeps each bin in a list once.
 //
 // Kernel A (detect) registers each singleton in an open-addressing hash
 // table keyed by a 64-bit hash ("tag") of k: CAS claims a slot, atomicMin /
 // atomicMax on (pass+1) << 32 | bin record the first and the last detection,
 // and a record (time, rho) is pushed on the slot's lock-free list. Kernel B
-// (peel) checks that every detection's k equals its slot's key (a 64-bit tag
-// collision sets an error bit instead of merging two keys), writes the key
-// of new slots, and lets the last detection of each key peel it with FP64
-// atomics. No kernel waits on another thread's progress.
+// (peel, one wave per singleton) checks that the detection's k equals its
+// slot's key (a 64-bit tag collision sets an error bit instead of merging
+// two keys), writes the key of new slots, and lets the last detection of each
+// key peel it from every group with coalesced FP64 atomics over the
+// contiguous column. No kernel waits on another thread's progress.
 #include "qsft_common.h"
 
 namespace qsft {
 
-constexpr int kPeelThreads = 128;
+constexpr int kWaves = 4;             // waves per block of the per-bin kernels
+constexpr int kBinThreads = 64 * kWaves;
+constexpr int kBinsPerWave = 16;      // first-pass bins per wave (grid sizing)
+constexpr int kRowsPerLane = 4;       // P <= 256
 constexpr int kSlotThreads = 256;
 constexpr uint32_t kNil = 0xFFFFFFFFu;
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+__device__ __forceinline__ int wave_sum_i(int v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
 
 // numpy's remainder(x, 2 pi) for finite x: fmod, then shift negatives up.
 __device__ __forceinline__ double rem_2pi(double x) {
@@ -47,27 +69,18 @@ __device__ __forceinline__ double rem_2pi(double x) {
     return m;
 }
 
-// <row, k> over n_pad digits; k lives in LDS as words ks[w * kPeelThreads + tid].
-__device__ __forceinline__ uint32_t dot_k(const int8_t* __restrict__ row, const uint32_t* ks, int nw) {
-    const int tid = threadIdx.x;
+// <row, k> over nw 4-digit words; k is the wave's LDS copy (broadcast reads).
+__device__ __forceinline__ uint32_t dot_row(const int8_t* __restrict__ row, const uint32_t* k, int nw) {
+    const int* r = reinterpret_cast<const int*>(row);
     int acc = 0;
-    const uint32_t* r32 = reinterpret_cast<const uint32_t*>(row);
-    for (int w = 0; w < nw; ++w) acc = __builtin_amdgcn_sdot4((int)r32[w], (int)ks[w * kPeelThreads + tid], acc, false);
+    for (int w = 0; w < nw; ++w) acc = __builtin_amdgcn_sdot4(r[w], (int)k[w], acc, false);
     return (uint32_t)acc;
 }
 
-__device__ __forceinline__ void set_digit(uint32_t* ks, int i, uint32_t v) {
-    const int tid = threadIdx.x;
-    uint32_t* w = ks + (i >> 2) * kPeelThreads + tid;
-    const int sh = 8 * (i & 3);
-    *w = (*w & ~(0xFFu << sh)) | ((v & 0xFFu) << sh);
-}
-
-__device__ __forceinline__ uint64_t hash_k(const uint32_t* ks, int nw) {
-    const int tid = threadIdx.x;
+__device__ __forceinline__ uint64_t hash_words(const uint32_t* k, int nw) {
     uint64_t h = 0xCBF29CE484222325ull;
     for (int w = 0; w < nw; ++w) {
-        h ^= ks[w * kPeelThreads + tid];
+        h ^= k[w];
         h *= 0x100000001B3ull;
     }
     return mix64(h);
@@ -78,6 +91,22 @@ __device__ __forceinline__ void fill_wtable_d(double2* tab, int q) {
         double s, c;
         sincospi(2.0 * (double)r / (double)q, &s, &c);
         tab[r] = make_double2(c, s);
+    }
+}
+
+// LDS writes by one lane, read by other lanes of the same wave.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+// Put bin t in the scan list of pass it + 1 (once).
+__device__ __forceinline__ void enlist(const qsft_peel_desc& d, int it, int t) {
+    if (atomicMax(&d.mark[t], it + 1) < it + 1) {
+        int32_t* next = ((it + 1) & 1) ? d.list1 : d.list0;
+        const uint32_t pos = atomicAdd(&d.ctr[it * QSFT_CTR_ITER_STRIDE + 4], 1u);
+        next[pos] = t;
     }
 }
 
@@ -96,13 +125,13 @@ __global__ __launch_bounds__(kSlotThreads) void k_peel_init(const qsft_peel_desc
         d.head[h] = kNil;
         d.count[h] = 0;
     }
+    if (h < d.C * d.Q) d.mark[h] = 0;
 }
 
-__global__ __launch_bounds__(kPeelThreads) void k_peel_detect(const qsft_peel_desc* __restrict__ descs,
-                                                              const int32_t* __restrict__ pre, int n_inst,
-                                                              int it) {
+__global__ __launch_bounds__(kBinThreads) void k_peel_detect(const qsft_peel_desc* __restrict__ descs,
+                                                             const int32_t* __restrict__ pre, int n_inst, int it) {
     __shared__ double2 tab[QSFT_MAX_Q];
-    __shared__ uint32_t ks[(QSFT_MAX_N / 4) * kPeelThreads];
+    __shared__ uint32_t kws[kWaves][QSFT_MAX_N / 4];
     const int inst = find_inst(pre, n_inst, blockIdx.x);
     const qsft_peel_desc& d = descs[inst];
     uint32_t* __restrict__ ctr = d.ctr;
@@ -110,159 +139,177 @@ __global__ __launch_bounds__(kPeelThreads) void k_peel_detect(const qsft_peel_de
     if (ctr[it * QSFT_CTR_ITER_STRIDE + 3] == 0) return;          // loop already stopped
     const int q = d.q, n = d.n, b = d.b, C = d.C, P = d.P, Q = d.Q, n_pad = d.n_pad;
     const int nw = n_pad >> 2;
-    const int tid = threadIdx.x;
-    const int t = (blockIdx.x - pre[inst]) * kPeelThreads + tid;
     fill_wtable_d(tab, q);
-    for (int w = 0; w < nw; ++w) ks[w * kPeelThreads + tid] = 0;
     __syncthreads();
-    if (t >= C * Q) return;
 
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    uint32_t* kk = kws[wave];
+    int8_t* kb8 = reinterpret_cast<int8_t*>(kk);
+    const int CQ = C * Q;
+    const int count = (it == 0) ? CQ : (int)ctr[(it - 1) * QSFT_CTR_ITER_STRIDE + 4];
+    const int32_t* __restrict__ list = (it == 0) ? nullptr : ((it & 1) ? d.list1 : d.list0);
+    const int waves_inst = (pre[inst + 1] - pre[inst]) * kWaves;
     const ModQ mq((uint32_t)q);
-    const int c = t / Q, j = t - c * Q;
-    const double2* __restrict__ col = reinterpret_cast<const double2*>(d.U) + (size_t)c * P * Q + j;
     const double thr = d.cutoff * (double)P;
+    const double twopi = 6.283185307179586;
 
-    double sre = 0.0, sim = 0.0;
-    for (int p = 0; p < P; ++p) {
-        const double2 u = col[(size_t)p * Q];
-        sre += u.x * u.x;
-        sim += u.y * u.y;
-    }
-    double energy = sqrt(sre + sim);
-    energy = energy * energy;
-    int status = 0;                                  // 0 zeroton, 1 singleton, 2 multiton
-    double rho_x = 0.0, rho_y = 0.0;
-    if (energy > thr) {
-        const double twopi = 6.283185307179586;
-        if (d.method == QSFT_DETECT_NSO1) {
-            const int p1 = d.p1;
-            const int R = P / p1;
-            const double step = twopi / (double)q;   // 2 * np.pi / q
-            for (int i = 1; i < p1; ++i) {
-                double zr = 0.0, zi = 0.0;
-                for (int r = 0; r < R; ++r) {
-                    const double2 u0 = col[(size_t)(r * p1) * Q];
-                    const double2 ui = col[(size_t)(r * p1 + i) * Q];
-                    // u0 * conj(ui), as numpy forms the complex product
-                    const double pr = __dsub_rn(__dmul_rn(u0.x, ui.x), __dmul_rn(u0.y, -ui.y));
-                    const double pi_ = __dadd_rn(__dmul_rn(u0.x, -ui.y), __dmul_rn(u0.y, ui.x));
-                    zr = __dadd_rn(zr, pr);
-                    zi = __dadd_rn(zi, pi_);
+    for (int e = (blockIdx.x - pre[inst]) * kWaves + wave; e < count; e += waves_inst) {
+        const int t = list ? list[e] : e;
+        const int c = t / Q, j = t - c * Q;
+        const double2* __restrict__ col = reinterpret_cast<const double2*>(d.U) + (size_t)t * P;
+        double2 u[kRowsPerLane];
+        double sre = 0.0, sim = 0.0;
+#pragma unroll
+        for (int r = 0; r < kRowsPerLane; ++r) {
+            const int p = lane + 64 * r;
+            u[r] = (p < P) ? col[p] : make_double2(0.0, 0.0);
+            sre += u[r].x * u[r].x;
+            sim += u[r].y * u[r].y;
+        }
+        sre = wave_sum(sre);
+        sim = wave_sum(sim);
+        double energy = sqrt(sre + sim);
+        energy = energy * energy;
+        if (!(energy > thr)) continue;                             // zeroton (wave-uniform)
+
+        // --- k: one coordinate per lane
+        if (lane < nw) kk[lane] = 0;
+        wave_lds_sync();
+        for (int i0 = 1; i0 <= n; i0 += 64) {
+            const int i = i0 + lane;
+            if (i <= n) {
+                uint32_t digit;
+                if (d.method == QSFT_DETECT_NSO1) {
+                    const int p1 = d.p1, R = P / p1;
+                    double zr = 0.0, zi = 0.0;
+                    for (int r = 0; r < R; ++r) {
+                        const double2 u0 = col[r * p1];
+                        const double2 ui = col[r * p1 + i];
+                        // u0 * conj(ui), as numpy forms the complex product
+                        const double pr = __dsub_rn(__dmul_rn(u0.x, ui.x), __dmul_rn(u0.y, -ui.y));
+                        const double pi_ = __dadd_rn(__dmul_rn(u0.x, -ui.y), __dmul_rn(u0.y, ui.x));
+                        zr = __dadd_rn(zr, pr);
+                        zi = __dadd_rn(zi, pi_);
+                    }
+                    zr = __ddiv_rn(zr, (double)R);
+                    zi = __ddiv_rn(zi, (double)R);
+                    const double th = rem_2pi(atan2(zi, zr));
+                    const double step = twopi / (double)q;         // 2 * np.pi / q
+                    int best = 0;
+                    double bestv = fabs(__dsub_rn(0.0, th));
+                    for (int tt = 1; tt <= q; ++tt) {
+                        const double v = fabs(__dsub_rn(__dmul_rn(step, (double)tt), th));
+                        if (v < bestv) { bestv = v; best = tt; }
+                    }
+                    digit = (uint32_t)(best % q);
+                } else {
+                    const double2 v0 = col[0];
+                    const double2 vi = col[i];
+                    const double a0 = atan2(v0.y, v0.x), ai = atan2(vi.y, vi.x);
+                    const double v = __ddiv_rn(__dmul_rn((double)q, __dsub_rn(ai, a0)), twopi);
+                    long long kq = (long long)rint(v) % q;
+                    if (kq < 0) kq += q;
+                    digit = (uint32_t)kq;
                 }
-                zr = __ddiv_rn(zr, (double)R);
-                zi = __ddiv_rn(zi, (double)R);
-                const double th = rem_2pi(atan2(zi, zr));
-                int best = 0;
-                double bestv = fabs(__dsub_rn(0.0, th));
-                for (int tt = 1; tt <= q; ++tt) {
-                    const double v = fabs(__dsub_rn(__dmul_rn(step, (double)tt), th));
-                    if (v < bestv) { bestv = v; best = tt; }
-                }
-                set_digit(ks, i - 1, (uint32_t)(best % q));
-            }
-        } else {
-            const double2 v0 = col[0];
-            const double a0 = atan2(v0.y, v0.x);
-            for (int i = 1; i <= n; ++i) {
-                const double2 vi = col[(size_t)i * Q];
-                const double ai = atan2(vi.y, vi.x);
-                const double v = __ddiv_rn(__dmul_rn((double)q, __dsub_rn(ai, a0)), twopi);
-                long long kk = (long long)rint(v);
-                kk %= q;
-                if (kk < 0) kk += q;
-                set_digit(ks, i - 1, (uint32_t)kk);
+                kb8[i - 1] = (int8_t)digit;
             }
         }
-        // signature, rho
+        wave_lds_sync();
+
+        // --- signature, rho, residual (rows across lanes)
         const int8_t* __restrict__ Dc = d.D + (size_t)c * P * n_pad;
+        double2 w[kRowsPerLane];
         double rr = 0.0, ri = 0.0;
-        for (int p = 0; p < P; ++p) {
-            const double2 w = tab[mq(dot_k(Dc + (size_t)p * n_pad, ks, nw))];
-            const double2 u = col[(size_t)p * Q];
-            rr += w.x * u.x + w.y * u.y;              // conj(w) * u
-            ri += w.x * u.y - w.y * u.x;
+#pragma unroll
+        for (int r = 0; r < kRowsPerLane; ++r) {
+            const int p = lane + 64 * r;
+            w[r] = make_double2(0.0, 0.0);
+            if (p < P) {
+                w[r] = tab[mq(dot_row(Dc + (size_t)p * n_pad, kk, nw))];
+                rr += w[r].x * u[r].x + w[r].y * u[r].y;           // conj(w) * u
+                ri += w[r].x * u[r].y - w[r].y * u[r].x;
+            }
         }
-        rho_x = __ddiv_rn(rr, (double)P);
-        rho_y = __ddiv_rn(ri, (double)P);
+        rr = wave_sum(rr);
+        ri = wave_sum(ri);
+        const double rho_x = __ddiv_rn(rr, (double)P), rho_y = __ddiv_rn(ri, (double)P);
         double qre = 0.0, qim = 0.0;
-        for (int p = 0; p < P; ++p) {
-            const double2 w = tab[mq(dot_k(Dc + (size_t)p * n_pad, ks, nw))];
-            const double2 u = col[(size_t)p * Q];
-            const double sx = __dsub_rn(__dmul_rn(rho_x, w.x), __dmul_rn(rho_y, w.y));
-            const double sy = __dadd_rn(__dmul_rn(rho_x, w.y), __dmul_rn(rho_y, w.x));
-            const double ex = __dsub_rn(u.x, sx), ey = __dsub_rn(u.y, sy);
-            qre += ex * ex;
-            qim += ey * ey;
+#pragma unroll
+        for (int r = 0; r < kRowsPerLane; ++r) {
+            const int p = lane + 64 * r;
+            if (p < P) {
+                const double sx = __dsub_rn(__dmul_rn(rho_x, w[r].x), __dmul_rn(rho_y, w[r].y));
+                const double sy = __dadd_rn(__dmul_rn(rho_x, w[r].y), __dmul_rn(rho_y, w[r].x));
+                const double ex = __dsub_rn(u[r].x, sx), ey = __dsub_rn(u[r].y, sy);
+                qre += ex * ex;
+                qim += ey * ey;
+            }
         }
+        qre = wave_sum(qre);
+        qim = wave_sum(qim);
         double res = sqrt(qre + qim);
         res = res * res;
-        // bin match: M_c^T k mod q == digits of j (MSB first)
-        const int8_t* __restrict__ Mc = d.Mt + (size_t)c * b * n_pad;
-        bool match = true;
-        int rem = j, pw = Q;
-        for (int tt = 0; tt < b; ++tt) {
-            pw /= q;
-            const int dig = rem / pw;
-            rem -= dig * pw;
-            match = match && ((int)mq(dot_k(Mc + (size_t)tt * n_pad, ks, nw)) == dig);
-        }
-        status = (match && res <= thr) ? 1 : 2;
-    }
 
-    int32_t* slot_bin = d.slot_bin;
-    if (status == 2) {
-        atomicAdd(&ctr[it * QSFT_CTR_ITER_STRIDE + 1], 1u);
-        slot_bin[t] = -1;
-        return;
+        // --- bin match: M_c^T k mod q == digits of j (MSB first), one digit per lane
+        bool ok = true;
+        if (lane < b) {
+            int pw = 1;
+            for (int tt = lane + 1; tt < b; ++tt) pw *= q;
+            const int dig = (j / pw) % q;
+            ok = (int)mq(dot_row(d.Mt + ((size_t)c * b + lane) * n_pad, kk, nw)) == dig;
+        }
+        const bool match = __all(ok);
+        const bool single = match && res <= thr;
+
+        if (lane == 0) {
+            enlist(d, it, t);                                     // non-zeroton: rescan next pass
+            atomicAdd(&ctr[it * QSFT_CTR_ITER_STRIDE + (single ? 0 : 1)], 1u);
+        }
+        if (!single) continue;
+        if (lane < nw) reinterpret_cast<uint32_t*>(d.kbin + (size_t)t * n_pad)[lane] = kk[lane];
+        if (lane == 0) {
+            d.rho_bin[2 * (size_t)t] = rho_x;
+            d.rho_bin[2 * (size_t)t + 1] = rho_y;
+            const uint64_t hv = hash_words(kk, nw);
+            const uint64_t tg = hv | 1ull;
+            const uint32_t mask = (1u << d.hcap_log2) - 1u;
+            uint32_t pos = (uint32_t)(hv >> 32) & mask;
+            int slot = -1;
+            for (uint32_t probe = 0; probe <= mask; ++probe) {
+                const unsigned long long old =
+                    atomicCAS((unsigned long long*)&d.tag[pos], 0ull, (unsigned long long)tg);
+                if (old == 0ull || old == tg) { slot = (int)pos; break; }
+                pos = (pos + 1) & mask;
+            }
+            if (slot < 0) {
+                atomicOr(&ctr[QSFT_CTR_ERROR], (uint32_t)QSFT_DEVERR_TABLE_FULL);
+            } else {
+                d.slot_bin[t] = slot;
+                const uint64_t tm = ((uint64_t)(it + 1) << 32) | (uint32_t)t;
+                atomicMin((unsigned long long*)&d.owner[slot], (unsigned long long)tm);
+                const unsigned long long prev = atomicMax((unsigned long long*)&d.last[slot], (unsigned long long)tm);
+                if ((prev >> 32) != (unsigned long long)(it + 1)) atomicAdd(&ctr[it * QSFT_CTR_ITER_STRIDE + 2], 1u);
+                const uint32_t rix = atomicAdd(&ctr[QSFT_CTR_LOG], 1u);
+                if ((uint64_t)rix >= (uint64_t)d.max_iter * (uint64_t)CQ) {   // cannot happen: one record per bin per pass
+                    atomicOr(&ctr[QSFT_CTR_ERROR], (uint32_t)QSFT_DEVERR_BAD_SHAPE);
+                } else {
+                    d.rec_time[rix] = tm;
+                    d.rec_rho[2 * (size_t)rix] = rho_x;
+                    d.rec_rho[2 * (size_t)rix + 1] = rho_y;
+                    d.rec_next[rix] = atomicExch(&d.head[slot], rix);
+                    atomicAdd(&d.count[slot], 1u);
+                    const uint32_t sp = atomicAdd(&ctr[it * QSFT_CTR_ITER_STRIDE + 5], 1u);
+                    d.slist[sp] = t;
+                }
+            }
+        }
     }
-    if (status == 0) {
-        slot_bin[t] = -1;
-        return;
-    }
-    atomicAdd(&ctr[it * QSFT_CTR_ITER_STRIDE + 0], 1u);
-    // record k and rho of this singleton bin
-    uint32_t* kdst = reinterpret_cast<uint32_t*>(d.kbin + (size_t)t * n_pad);
-    for (int w = 0; w < nw; ++w) kdst[w] = ks[w * kPeelThreads + tid];
-    d.rho_bin[2 * (size_t)t] = rho_x;
-    d.rho_bin[2 * (size_t)t + 1] = rho_y;
-    // claim or find the slot of k
-    const uint64_t hv = hash_k(ks, nw);
-    const uint64_t tg = hv | 1ull;
-    const uint32_t mask = (1u << d.hcap_log2) - 1u;
-    uint32_t pos = (uint32_t)(hv >> 32) & mask;
-    int slot = -1;
-    for (uint32_t probe = 0; probe <= mask; ++probe) {
-        const unsigned long long old = atomicCAS((unsigned long long*)&d.tag[pos], 0ull, (unsigned long long)tg);
-        if (old == 0ull || old == tg) { slot = (int)pos; break; }
-        pos = (pos + 1) & mask;
-    }
-    if (slot < 0) {
-        atomicOr(&ctr[QSFT_CTR_ERROR], (uint32_t)QSFT_DEVERR_TABLE_FULL);
-        slot_bin[t] = -1;
-        return;
-    }
-    slot_bin[t] = slot;
-    const uint64_t tm = ((uint64_t)(it + 1) << 32) | (uint32_t)t;
-    atomicMin((unsigned long long*)&d.owner[slot], (unsigned long long)tm);
-    const unsigned long long prev = atomicMax((unsigned long long*)&d.last[slot], (unsigned long long)tm);
-    if ((prev >> 32) != (unsigned long long)(it + 1)) atomicAdd(&ctr[it * QSFT_CTR_ITER_STRIDE + 2], 1u);
-    const uint32_t r = atomicAdd(&ctr[QSFT_CTR_LOG], 1u);
-    if ((uint64_t)r >= (uint64_t)d.max_iter * (uint64_t)(C * Q)) {   // cannot happen: one record per bin per pass
-        atomicOr(&ctr[QSFT_CTR_ERROR], (uint32_t)QSFT_DEVERR_BAD_SHAPE);
-        return;
-    }
-    d.rec_time[r] = tm;
-    d.rec_rho[2 * (size_t)r] = rho_x;
-    d.rec_rho[2 * (size_t)r + 1] = rho_y;
-    d.rec_next[r] = atomicExch(&d.head[slot], r);
-    atomicAdd(&d.count[slot], 1u);
 }
 
-__global__ __launch_bounds__(kPeelThreads) void k_peel_apply(const qsft_peel_desc* __restrict__ descs,
-                                                             const int32_t* __restrict__ pre, int n_inst,
-                                                             int it) {
+__global__ __launch_bounds__(kBinThreads) void k_peel_apply(const qsft_peel_desc* __restrict__ descs,
+                                                            const int32_t* __restrict__ pre, int n_inst, int it) {
     __shared__ double2 tab[QSFT_MAX_Q];
-    __shared__ uint32_t ks[(QSFT_MAX_N / 4) * kPeelThreads];
+    __shared__ uint32_t kws[kWaves][QSFT_MAX_N / 4];
     const int inst = find_inst(pre, n_inst, blockIdx.x);
     const qsft_peel_desc& d = descs[inst];
     uint32_t* __restrict__ ctr = d.ctr;
@@ -286,48 +333,60 @@ __global__ __launch_bounds__(kPeelThreads) void k_peel_apply(const qsft_peel_des
     if (!active) return;
     const int q = d.q, b = d.b, C = d.C, P = d.P, Q = d.Q, n_pad = d.n_pad;
     const int nw = n_pad >> 2;
-    const int tid = threadIdx.x;
-    const int t = local * kPeelThreads + tid;
     fill_wtable_d(tab, q);
     __syncthreads();
-    if (t >= C * Q) return;
-    const int slot = d.slot_bin[t];
-    if (slot < 0) return;
-    const uint64_t tm = ((uint64_t)(it + 1) << 32) | (uint32_t)t;
-    // bring k into LDS and check it against the slot's key
-    const uint32_t* ksrc = reinterpret_cast<const uint32_t*>(d.kbin + (size_t)t * n_pad);
-    for (int w = 0; w < nw; ++w) ks[w * kPeelThreads + tid] = ksrc[w];
-    const uint64_t own = d.owner[slot];
-    uint32_t* kslot = reinterpret_cast<uint32_t*>(d.key + (size_t)slot * n_pad);
-    if ((own >> 32) == (uint64_t)(it + 1)) {
-        const uint32_t ot = (uint32_t)own;
-        if (own == tm) {
-            for (int w = 0; w < nw; ++w) kslot[w] = ks[w * kPeelThreads + tid];
-        } else {
-            const uint32_t* kown = reinterpret_cast<const uint32_t*>(d.kbin + (size_t)ot * n_pad);
-            for (int w = 0; w < nw; ++w)
-                if (kown[w] != ks[w * kPeelThreads + tid]) atomicOr(&ctr[QSFT_CTR_ERROR], (uint32_t)QSFT_DEVERR_TAG_COLLISION);
-        }
-    } else {
-        for (int w = 0; w < nw; ++w)
-            if (kslot[w] != ks[w * kPeelThreads + tid]) atomicOr(&ctr[QSFT_CTR_ERROR], (uint32_t)QSFT_DEVERR_TAG_COLLISION);
-    }
-    if (d.last[slot] != tm) return;
-    // the last detection of this key in this pass peels it from every group
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    uint32_t* kk = kws[wave];
+    const int count = (int)ctr[it * QSFT_CTR_ITER_STRIDE + 5];
+    const int waves_inst = (pre[inst + 1] - pre[inst]) * kWaves;
     const ModQ mq((uint32_t)q);
-    const double rx = d.rho_bin[2 * (size_t)t], ry = d.rho_bin[2 * (size_t)t + 1];
-    for (int l = 0; l < C; ++l) {
-        const int8_t* __restrict__ Ml = d.Mt + (size_t)l * b * n_pad;
-        int jl = 0;
-        for (int tt = 0; tt < b; ++tt) jl = jl * q + (int)mq(dot_k(Ml + (size_t)tt * n_pad, ks, nw));
-        const int8_t* __restrict__ Dl = d.D + (size_t)l * P * n_pad;
-        double* __restrict__ Ul = d.U + 2 * ((size_t)l * P * Q + jl);
-        for (int p = 0; p < P; ++p) {
-            const double2 w = tab[mq(dot_k(Dl + (size_t)p * n_pad, ks, nw))];
-            const double sx = __dsub_rn(__dmul_rn(rx, w.x), __dmul_rn(ry, w.y));
-            const double sy = __dadd_rn(__dmul_rn(rx, w.y), __dmul_rn(ry, w.x));
-            unsafeAtomicAdd(Ul + 2 * (size_t)p * Q, -sx);
-            unsafeAtomicAdd(Ul + 2 * (size_t)p * Q + 1, -sy);
+
+    for (int e = local * kWaves + wave; e < count; e += waves_inst) {
+        const int t = d.slist[e];
+        const int slot = d.slot_bin[t];
+        const uint64_t tm = ((uint64_t)(it + 1) << 32) | (uint32_t)t;
+        const uint32_t* ksrc = reinterpret_cast<const uint32_t*>(d.kbin + (size_t)t * n_pad);
+        if (lane < nw) kk[lane] = ksrc[lane];
+        wave_lds_sync();
+        // the detection's k must equal its slot's key
+        const uint64_t own = d.owner[slot];
+        uint32_t* kslot = reinterpret_cast<uint32_t*>(d.key + (size_t)slot * n_pad);
+        bool same = true;
+        if ((own >> 32) == (uint64_t)(it + 1)) {
+            if (own == tm) {
+                if (lane < nw) kslot[lane] = kk[lane];
+            } else if (lane < nw) {
+                same = reinterpret_cast<const uint32_t*>(d.kbin + (size_t)(uint32_t)own * n_pad)[lane] == kk[lane];
+            }
+        } else if (lane < nw) {
+            same = kslot[lane] == kk[lane];
+        }
+        if (!__all(same) && lane == 0) atomicOr(&ctr[QSFT_CTR_ERROR], (uint32_t)QSFT_DEVERR_TAG_COLLISION);
+        if (d.last[slot] != tm) continue;
+        // the last detection of this key in this pass peels it from every group
+        const double rx = d.rho_bin[2 * (size_t)t], ry = d.rho_bin[2 * (size_t)t + 1];
+        for (int l = 0; l < C; ++l) {
+            int hd = 0;
+            if (lane < b) {
+                int pw = 1;
+                for (int tt = lane + 1; tt < b; ++tt) pw *= q;
+                hd = (int)mq(dot_row(d.Mt + ((size_t)l * b + lane) * n_pad, kk, nw)) * pw;
+            }
+            const int jl = wave_sum_i(hd);
+            const int8_t* __restrict__ Dl = d.D + (size_t)l * P * n_pad;
+            double* __restrict__ Ucol = d.U + 2 * ((size_t)l * Q + jl) * P;
+#pragma unroll
+            for (int r = 0; r < kRowsPerLane; ++r) {
+                const int p = lane + 64 * r;
+                if (p < P) {
+                    const double2 wv = tab[mq(dot_row(Dl + (size_t)p * n_pad, kk, nw))];
+                    const double sx = __dsub_rn(__dmul_rn(rx, wv.x), __dmul_rn(ry, wv.y));
+                    const double sy = __dadd_rn(__dmul_rn(rx, wv.y), __dmul_rn(ry, wv.x));
+                    unsafeAtomicAdd(Ucol + 2 * p, -sx);
+                    unsafeAtomicAdd(Ucol + 2 * p + 1, -sy);
+                }
+            }
+            if (lane == 0) enlist(d, it, l * Q + jl);              // the peeled bin is rescanned
         }
     }
 }
@@ -445,15 +504,16 @@ PeelLayout peel_layout(long long CQ, int n_pad, int hl2, int max_iter, int out_c
     const size_t H = (size_t)1 << hl2;
     const size_t L = (size_t)max_iter * CQ;
     const size_t sizes[] = {
-        (size_t)CQ * n_pad,      // kbin
-        (size_t)CQ * 16,         // rho_bin
-        (size_t)CQ * 4,          // slot_bin
-        H * 8, H * 8, H * 8,     // tag owner last
-        H * 4, H * 4,            // head count
-        H * (size_t)n_pad,       // key
-        L * 8, L * 16, L * 4,    // rec_time rec_rho rec_next
-        QSFT_CTR_WORDS * 4,      // ctr
-        (size_t)out_cap * n_pad, (size_t)out_cap * 16, (size_t)out_cap * 8,  // outputs
+        (size_t)CQ * n_pad,      // 0 kbin
+        (size_t)CQ * 16,         // 1 rho_bin
+        (size_t)CQ * 4,          // 2 slot_bin
+        H * 8, H * 8, H * 8,     // 3-5 tag owner last
+        H * 4, H * 4,            // 6-7 head count
+        H * (size_t)n_pad,       // 8 key
+        L * 8, L * 16, L * 4,    // 9-11 rec_time rec_rho rec_next
+        QSFT_CTR_WORDS * 4,      // 12 ctr
+        (size_t)out_cap * n_pad, (size_t)out_cap * 16, (size_t)out_cap * 8,  // 13-15 outputs
+        (size_t)CQ * 4, (size_t)CQ * 4, (size_t)CQ * 4, (size_t)CQ * 4,      // 16-19 mark list0 list1 slist
     };
     PeelLayout lay{};
     size_t o = 0;
@@ -480,10 +540,11 @@ extern "C" int qsft_peel_bind(qsft_peel_desc* h, int q, int n, int b, int C, int
                               int max_iter, double cutoff, int out_cap, const int8_t* Mt, const int8_t* D,
                               double* U, void* ws, size_t ws_bytes, int n_pad) {
     if (!h || !ws || q < 2 || q > QSFT_MAX_Q || n < 1 || n > QSFT_MAX_N || b < 1 || b > QSFT_MAX_B || C < 1 ||
-        C > QSFT_MAX_C || P < 1 || p1 < 2 || P % p1 || max_iter < 1 || max_iter > 15 || n_pad % 64 ||
-        n_pad < n || n_pad > QSFT_MAX_N || (method != QSFT_DETECT_NSO1 && method != QSFT_DETECT_NOISELESS) ||
+        C > QSFT_MAX_C || P < 1 || P > 64 * kRowsPerLane || p1 < 2 || P % p1 || max_iter < 1 || max_iter > 15 ||
+        n_pad % 64 || n_pad < n || n_pad > QSFT_MAX_N ||
+        (method != QSFT_DETECT_NSO1 && method != QSFT_DETECT_NOISELESS) ||
         (method == QSFT_DETECT_NOISELESS && P != n + 1) || (method == QSFT_DETECT_NSO1 && p1 != n + 1)) {
-        qsft_set_error("qsft_peel_bind: bad shape");
+        qsft_set_error("qsft_peel_bind: bad shape (P <= 256, p1 = n + 1, noiseless needs P = n + 1)");
         return QSFT_ERR_ARG;
     }
     const long long Qll = ipow(q, b);
@@ -519,6 +580,10 @@ extern "C" int qsft_peel_bind(qsft_peel_desc* h, int q, int n, int b, int C, int
     d.out_k = reinterpret_cast<int8_t*>(base + lay.off[13]);
     d.out_val = reinterpret_cast<double*>(base + lay.off[14]);
     d.out_order = reinterpret_cast<uint64_t*>(base + lay.off[15]);
+    d.mark = reinterpret_cast<int32_t*>(base + lay.off[16]);
+    d.list0 = reinterpret_cast<int32_t*>(base + lay.off[17]);
+    d.list1 = reinterpret_cast<int32_t*>(base + lay.off[18]);
+    d.slist = reinterpret_cast<int32_t*>(base + lay.off[19]);
     *h = d;
     return QSFT_OK;
 }
@@ -526,12 +591,15 @@ extern "C" int qsft_peel_bind(qsft_peel_desc* h, int q, int n, int b, int C, int
 extern "C" int qsft_peel_bin_blocks(const qsft_peel_desc* h) {
     if (!h) return -1;
     const long long CQ = (long long)h->C * h->Q;
-    return (int)((CQ + kPeelThreads - 1) / kPeelThreads);
+    const long long per = (long long)kWaves * kBinsPerWave;
+    return (int)((CQ + per - 1) / per);
 }
 
 extern "C" int qsft_peel_slot_blocks(const qsft_peel_desc* h) {
     if (!h) return -1;
-    return (int)(((1ll << h->hcap_log2) + kSlotThreads - 1) / kSlotThreads);
+    const long long H = 1ll << h->hcap_log2;
+    const long long CQ = (long long)h->C * h->Q;
+    return (int)(((H > CQ ? H : CQ) + kSlotThreads - 1) / kSlotThreads);
 }
 
 extern "C" int qsft_peel_init(const qsft_peel_desc* d, const int32_t* pre_slots, int n_inst, int blocks_slots,
@@ -548,10 +616,10 @@ extern "C" int qsft_peel_iteration(const qsft_peel_desc* d, const int32_t* pre_b
         return QSFT_ERR_ARG;
     }
     hipStream_t st = (hipStream_t)stream;
-    hipLaunchKernelGGL(k_peel_detect, dim3(blocks_bins), dim3(kPeelThreads), 0, st, d, pre_bins, n_inst, it);
+    hipLaunchKernelGGL(k_peel_detect, dim3(blocks_bins), dim3(kBinThreads), 0, st, d, pre_bins, n_inst, it);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return qsft_hip_status(e, "qsft_peel_iteration(detect)");
-    hipLaunchKernelGGL(k_peel_apply, dim3(blocks_bins), dim3(kPeelThreads), 0, st, d, pre_bins, n_inst, it);
+    hipLaunchKernelGGL(k_peel_apply, dim3(blocks_bins), dim3(kBinThreads), 0, st, d, pre_bins, n_inst, it);
     return qsft_hip_status(hipGetLastError(), "qsft_peel_iteration(apply)");
 }
 

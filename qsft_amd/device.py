@@ -69,6 +69,16 @@ def evaluate_cosets(q, n, b, Mt, D, Kd, a, precision):
     return x
 
 
+def evaluate_cosets_mfma(q, n, b, Mt, D, Kd, a64):
+    """The same samples through the matrix-core kernel -> complex64 [C][P][q^b]."""
+    C, P = Mt.shape[0], D.shape[1]
+    x = torch.empty((C, P, q ** b), dtype=torch.complex64, device=Mt.device)
+    ev = K.EvalMfma([dict(q=q, n=n, b=b, C=C, P=P, S=Kd.shape[0], Mt=Mt, D=D, K=Kd, a64=a64, x=x)])
+    ev.launch()
+    torch.cuda.current_stream().synchronize()      # the B-fragment workspace is released with `ev`
+    return x
+
+
 def query_digits(q, n, b, Mt, D):
     """Digits of every coset query -> int8 [C][P][q^b][n_pad] (stage (a) kernel)."""
     C, P, n_pad = Mt.shape[0], D.shape[1], Mt.shape[-1]

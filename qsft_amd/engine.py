@@ -97,9 +97,18 @@ def random_spec(q, n, S, b, C=3, R=2, snr_db=10.0, rng=None, seed=0, distinct=Tr
 
 
 class Batch:
-    def __init__(self, specs, precision="fp32", max_iter=15, device=None):
+    """eval_mode: "mfma" (matrix-core GEMM form, FP32 only; the default for
+    FP32) or "direct" (per-query int8 MFMA phase + VALU w-table accumulation,
+    FP32 or FP64; the plain path)."""
+
+    def __init__(self, specs, precision="fp32", max_iter=15, device=None, eval_mode=None):
         self.specs = list(specs)
         self.precision = precision
+        if eval_mode is None:
+            eval_mode = "mfma" if precision == "fp32" else "direct"
+        if eval_mode == "mfma" and precision != "fp32":
+            raise ValueError("the matrix-core evaluator produces FP32 samples")
+        self.eval_mode = eval_mode
         code, dt = dv.PRECISIONS[precision]
         dev = device if device is not None else dv.current_device()
         self.device = dev
@@ -111,21 +120,22 @@ class Batch:
             Mt = dv.Mt_tensor(sp.Ms, sp.b, n_pad, dev)
             Dd = dv.D_tensor([sp.D] * C, n_pad, dev)
             Kd = dv.digits_tensor(sp.locq.T, n_pad, dev)
-            a = dv.strengths_tensor(sp.strengths, precision, dev)
+            a = dv.strengths_tensor(sp.strengths, "fp64" if eval_mode == "mfma" else precision, dev)
             x = torch.empty((C, P, Q), dtype=dt, device=dev)
-            U = torch.empty((C, P, Q), dtype=torch.complex128, device=dev)
+            U = torch.empty((C, Q, P), dtype=torch.complex128, device=dev)      # bin-major for the peeler
             self.bufs.append(dict(Mt=Mt, D=Dd, K=Kd, a=a, x=x, U=U))
-            ev.append(dict(q=sp.q, n=sp.n, b=sp.b, C=C, P=P, S=sp.S, Mt=Mt, D=Dd, K=Kd, a=a, x=x))
+            ev.append(dict(q=sp.q, n=sp.n, b=sp.b, C=C, P=P, S=sp.S, Mt=Mt, D=Dd, K=Kd, a=a, a64=a, x=x,
+                           shared_d=True))
             df.append(dict(q=sp.q, b=sp.b, rows=C * P, inp=x, out=x))
             item = dict(Q=Q, C=C, R=P // (sp.n + 1), p1=sp.n + 1, src=x, src_rows=P,
-                        gsel=list(range(C)), rsel=list(range(P // (sp.n + 1))), U=U)
+                        gsel=list(range(C)), rsel=list(range(P // (sp.n + 1))), U=U, bin_major=True)
             if sp.noise_sd > 0:
                 item.update(sigma=float(sp.noise_sd / np.sqrt(Q)), seed=sp.seed)
             asm.append(item)
             pl.append(dict(q=sp.q, n=sp.n, b=sp.b, C=C, P=P, p1=sp.n + 1,
                            method=L.QSFT_DETECT_NSO1 if sp.method == "nso" else L.QSFT_DETECT_NOISELESS,
                            cutoff=sp.threshold(), Mt=Mt, D=Dd, U=U))
-        self.eval = K.EvalCosets(ev, code)
+        self.eval = K.EvalMfma(ev) if eval_mode == "mfma" else K.EvalCosets(ev, code)
         self.dft = K.Dft(df, code)
         self.asm = K.Assemble(asm)
         self.peel = K.PeelBatch(pl, max_iter=max_iter)
@@ -147,9 +157,19 @@ class Batch:
         mark("assemble")
         self.peel.run(host_loop=host_loop)
         mark("peel")
+        self.done = torch.cuda.Event()
+        self.done.record()
 
     def results(self, with_info=False):
         return self.peel.results(with_info=with_info)
+
+    def fetch(self, copy_stream):
+        """Packed rows of the last launch, copied on `copy_stream` once that launch
+        is done: (counts, offsets, rows). Work queued after it keeps running."""
+        return self.peel.fetch_packed(copy_stream=copy_stream, ready=self.done)
+
+    def unpack(self, packed, with_info=False):
+        return self.peel.results(with_info=with_info, packed=packed)
 
 
 def nmse(k, vals, spec):

@@ -197,8 +197,9 @@ class SubsampledSignal(Signal):
         return None
 
     def _transform_input(self, ret_num_subsample, ret_num_repeat, b):
-        """Device counterpart of get_MDU used by QSFT.transform: (Ms, Ds, U [C][P][Q]
-        complex128 on the device, Ts). The stored sub-transforms are not modified."""
+        """Device counterpart of get_MDU used by QSFT.transform: (Ms, Ds, U, Ts) with U
+        the working transform, complex128 bin-major [C][Q][P] on the device. The
+        stored sub-transforms are not modified."""
         gsel, rsel = self._select(ret_num_subsample, ret_num_repeat, b)
         C, R = len(gsel), len(rsel)
         p1 = self.Ds[0][0].shape[0]
@@ -206,10 +207,10 @@ class SubsampledSignal(Signal):
         Ms = [self.Ms[i][:, :b] for i in gsel]
         Ds = [[self.Ds[i][j] for j in rsel] for i in gsel]
         Ts = [[self.transformTimes[i][j][b] for j in rsel] for i in gsel]
-        U = torch.empty((C, R * p1, Q), dtype=torch.complex128, device=self._dev)
+        U = torch.empty((C, Q, R * p1), dtype=torch.complex128, device=self._dev)
         noise = self._transform_noise(C, R, p1, b)
         item = dict(Q=Q, C=C, R=R, p1=p1, src=self._U[b], src_rows=self._U[b].shape[1], gsel=gsel, rsel=rsel,
-                    U=U)
+                    U=U, bin_major=True)
         if isinstance(noise, torch.Tensor):
             item["noise"] = noise
         elif isinstance(noise, dict):

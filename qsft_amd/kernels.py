@@ -108,6 +108,58 @@ def eval_cosets(items, precision):
     return p
 
 
+def mfma_mode(q):
+    """0: q in {2, 4} (w^e exact in bf16); 1: other powers of two; 2: any other q."""
+    if q in (2, 4):
+        return 0
+    return 1 if is_pow2(q) else 2
+
+
+class EvalMfma:
+    """Matrix-core coset evaluation (FP32 out).
+
+    items: dicts q, n, b, C, P, S, Mt, D, K, a64 (complex128 [S]), x (complex64 [C][P][q^b]).
+    The B-fragment workspace is allocated here, one per instance.
+    """
+
+    def __init__(self, items):
+        groups = defaultdict(list)
+        for it in items:
+            groups[(it["Mt"].shape[-1] // 64, mfma_mode(it["q"]))].append(it)
+        self.launches = []
+        self.ws = []
+        for (nkb, mode), its in groups.items():
+            descs, prep, main = [], [], []
+            for it in its:
+                if it["x"].dtype != torch.complex64 or it["a64"].dtype != torch.complex128:
+                    raise TypeError("EvalMfma needs complex64 x and complex128 strengths")
+                C, P, S = it["C"], it["P"], it["S"]
+                shared = it.get("shared_d")
+                if shared is None:
+                    Dg = it["D"]
+                    shared = all(torch.equal(Dg[0], Dg[c]) for c in range(1, C))
+                ws = torch.empty(lib.qsft_eval_mfma_workspace_bytes(1 if shared else C, P, S), dtype=torch.uint8,
+                                 device=it["x"].device)
+                self.ws.append(ws)
+                d = L.EvalMfmaDesc(q=it["q"], n=it["n"], b=it["b"], C=C, P=P, S=S, n_pad=it["Mt"].shape[-1],
+                                   nsb=(S + 15) // 16, npt=(P + 7) // 8, shared_d=int(bool(shared)),
+                                   Mt=_ptr(it["Mt"]), D=_ptr(it["D"]), K=_ptr(it["K"]), a=_ptr(it["a64"]),
+                                   Bf=_ptr(ws), x=_ptr(it["x"]))
+                pb, mb = ctypes.c_int(), ctypes.c_int()
+                if lib.qsft_eval_mfma_blocks(ctypes.byref(d), ctypes.byref(pb), ctypes.byref(mb)) != 0:
+                    raise ValueError(f"bad mfma eval shape {[it[k] for k in ('q', 'n', 'b', 'C', 'P', 'S')]}")
+                descs.append(d)
+                prep.append(pb.value)
+                main.append(mb.value)
+            buf, dptr, (pp, pm), (tp, tm) = pack(descs, [prep, main], its[0]["x"].device)
+            self.launches.append((buf, dptr, pp, pm, len(descs), tp, tm, (nkb << 4) | (mode << 8)))
+
+    def launch(self):
+        for _, dptr, pp, pm, n, tp, tm, word in self.launches:
+            L.check(lib.qsft_eval_mfma(_vp(dptr), _vp(pp), _vp(pm), n, tp, tm, word, stream_handle()),
+                    "qsft_eval_mfma")
+
+
 class QueryDigits:
     """items: dicts q, n, b, C, P, Mt, D, m (int8 [C][P][q^b][n_pad] out)."""
 
@@ -190,13 +242,17 @@ class Dft:
                 descs.append(d)
             npass = max(d.n_pass for d in descs)
             pl = [[d.pass_blocks[p] if p < d.n_pass else 0 for d in descs] for p in range(npass)]
+            lds = [max(lib.qsft_dft_lds_bytes(ctypes.byref(d), p, precision) for d in descs if p < d.n_pass)
+                   for p in range(npass)]
+            if min(lds) < 0:
+                raise ValueError("DFT tile does not fit the LDS budget")
             buf, dptr, pres, tots = pack(descs, pl, its[0]["out"].device)
-            self.launches.append((buf, dptr, pres, tots, len(descs), precision | (q << 8)))
+            self.launches.append((buf, dptr, pres, tots, len(descs), precision | (q << 8), lds))
 
     def launch(self):
-        for _, dptr, pres, tots, n, word in self.launches:
+        for _, dptr, pres, tots, n, word, lds in self.launches:
             for p in range(len(pres)):
-                L.check(lib.qsft_dft_pass(_vp(dptr), _vp(pres[p]), n, tots[p], p, word, stream_handle()),
+                L.check(lib.qsft_dft_pass(_vp(dptr), _vp(pres[p]), n, tots[p], p, word, lds[p], stream_handle()),
                         "qsft_dft_pass")
 
 
@@ -216,7 +272,8 @@ class Assemble:
     def __init__(self, items):
         descs, blocks = [], []
         for it in items:
-            d = L.AssembleDesc(Q=it["Q"], C=it["C"], R=it["R"], p1=it["p1"], src_rows=it["src_rows"])
+            d = L.AssembleDesc(Q=it["Q"], C=it["C"], R=it["R"], p1=it["p1"], src_rows=it["src_rows"],
+                               bin_major=int(bool(it.get("bin_major", False))))
             src = it["src"]
             d.src_prec = L.QSFT_F32 if src.dtype == torch.complex64 else L.QSFT_F64
             d.src = _ptr(src)
@@ -261,7 +318,8 @@ def assemble(items):
 class PeelBatch:
     """Bound peel descriptors + workspaces for a list of instances.
 
-    items: dicts q, n, b, C, P, p1, method, cutoff, Mt, D, U.
+    items: dicts q, n, b, C, P, p1, method, cutoff, Mt, D, U (complex128,
+    bin-major [C][Q][P]).
     """
 
     def __init__(self, items, max_iter=15):
@@ -317,23 +375,44 @@ class PeelBatch:
         base = self.descs[i].ctr - self.ws[i].data_ptr()
         return int(self.ws[i][base + 4 * w: base + 4 * w + 4].view(torch.int32).item())
 
-    def fetch_packed(self):
-        """Copy the packed rows to the host: (counts, offsets, rows uint8 [total][row_stride])."""
+    def fetch_packed(self, copy_stream=None, ready=None):
+        """Copy the packed rows to the host: (counts, offsets, rows uint8 [total][row_stride]).
+
+        With `copy_stream` and `ready` (an event recorded after run()), the
+        copies wait only for that event on the copy stream, so later work
+        queued on the compute stream keeps the GPU busy meanwhile.
+        """
         n = len(self.descs)
-        self.hdr_host.copy_(self.packed[:self.hdr_bytes])          # synchronising copy
+        if copy_stream is None:
+            self.hdr_host.copy_(self.packed[:self.hdr_bytes])          # synchronising copy
+        else:
+            copy_stream.wait_event(ready)
+            with torch.cuda.stream(copy_stream):
+                self.hdr_host.copy_(self.packed[:self.hdr_bytes], non_blocking=True)
+            copy_stream.synchronize()
         hdr = self.hdr_host.numpy().view(np.int64)
         err = int(hdr[2 * n + 1])
         if err:
             raise L.QsftLibraryError(f"peel device error bits {err:#x} (1 table full, 2 tag collision, "
                                      "4 output full)")
         total = int(hdr[2 * n])
-        rows = self.packed[self.hdr_bytes:self.hdr_bytes + total * self.row_stride].cpu().numpy()
+        nbytes = total * self.row_stride
+        if getattr(self, "rows_host", None) is None or self.rows_host.numel() < nbytes:
+            self.rows_host = torch.empty(max(nbytes, 1 << 20), dtype=torch.uint8).pin_memory()
+        src = self.packed[self.hdr_bytes:self.hdr_bytes + nbytes]
+        if copy_stream is None:
+            self.rows_host[:nbytes].copy_(src)
+        else:
+            with torch.cuda.stream(copy_stream):
+                self.rows_host[:nbytes].copy_(src, non_blocking=True)
+            copy_stream.synchronize()
+        rows = self.rows_host[:nbytes].numpy().copy()
         return hdr[:n].copy(), hdr[n:2 * n].copy(), rows.reshape(total, self.row_stride)
 
-    def results(self, with_info=True):
+    def results(self, with_info=True, packed=None):
         """Per instance: (k int64 [m][n], values complex128 [m], order uint64 [m], info dict),
         rows sorted by first detection (the reference's dict insertion order)."""
-        counts, offs, rows = self.fetch_packed()
+        counts, offs, rows = self.fetch_packed() if packed is None else packed
         out = []
         for i, d in enumerate(self.descs):
             r = rows[offs[i]:offs[i] + counts[i]]
@@ -350,5 +429,7 @@ class PeelBatch:
         base = d.ctr - ws.data_ptr()
         ctr = ws[base: base + 4 * L.QSFT_CTR_WORDS].view(torch.int32).cpu().numpy().view(np.uint32)
         iters = int(ctr[L.QSFT_CTR_ITERS])
+        s = L.QSFT_CTR_ITER_STRIDE
         return {"iterations": iters, "records": int(ctr[L.QSFT_CTR_LOG]), "peeled": int(ctr[L.QSFT_CTR_PEELED]),
-                "trace": [(int(ctr[4 * t]), int(ctr[4 * t + 1]), int(ctr[4 * t + 2])) for t in range(iters)]}
+                "trace": [(int(ctr[s * t]), int(ctr[s * t + 1]), int(ctr[s * t + 2])) for t in range(iters)],
+                "scanned": [int(ctr[s * t + 4]) for t in range(iters)]}

@@ -46,7 +46,7 @@ class QSFT:
         raise NotImplementedError(f"reconstruct_method_channel={ch!r} (only 'nso' and 'identity')")
 
     def _working_input(self, signal):
-        """(Ms, Ds, U device complex128 [C][P][Q], Ts) for this call."""
+        """(Ms, Ds, U device complex128 bin-major [C][Q][P], Ts) for this call."""
         base_get = SubsampledSignal.get_MDU
         own = type(signal).get_MDU
         from .synthetic_signal import SyntheticSubsampledSignal
@@ -54,7 +54,8 @@ class QSFT:
             return signal._transform_input(self.num_subsample, self.num_repeat, self.b)
         # a user subclass overrides get_MDU with its own semantics: honour it
         Ms, Ds, Us, Ts = signal.get_MDU(self.num_subsample, self.num_repeat, self.b, trans_times=True)
-        U = np.array([np.vstack(u) for u in Us]).astype(np.complex128)
+        U = np.array([np.vstack(u) for u in Us]).astype(np.complex128)          # [C][P][Q]
+        U = np.ascontiguousarray(U.transpose(0, 2, 1))                        # -> [C][Q][P]
         return Ms, Ds, torch.from_numpy(U).to(dv.current_device()), Ts
 
     def transform(self, signal, verbosity=0, report=False, timing_verbose=False, **kwargs):
@@ -71,7 +72,7 @@ class QSFT:
         cutoff = kwargs.get("cutoff", cutoff)
         if verbosity >= 2:
             print("cutoff = ", cutoff, flush=True)
-        C, P, Q = U.shape
+        C, Q, P = U.shape
         dev = U.device
         n_pad = dv.n_pad_of(n)
         Mt = dv.Mt_tensor(Ms, b, n_pad, dev)

@@ -63,6 +63,51 @@ def test_eval_cosets_matches_oracle(q, n, b, S, C, R, precision):
             assert rel_err(x[c, p], ref) < tol, (c, p)
 
 
+MFMA_CASES = CASES + [(8, 30, 3, 77, 2, 2), (2, 250, 4, 33, 3, 1), (4, 64, 5, 129, 3, 2)]
+
+
+@pytest.mark.parametrize("q,n,b,S,C,R", MFMA_CASES)
+def test_eval_mfma_matches_oracle(q, n, b, S, C, R):
+    """Matrix-core evaluation (bf16x3 limbs, FP32 accumulation) against the
+    oracle's per-query FP64 evaluation, every row."""
+    print(f"eval_mfma q={q} n={n} b={b} S={S} C={C} R={R}", flush=True)
+    Ms, D, locq, a = _setup(q, n, b, S, C, R, seed=q * 1000 + n + 7)
+    dev = dv.current_device()
+    n_pad = dv.n_pad_of(n)
+    Mt = dv.Mt_tensor(Ms, b, n_pad, dev)
+    Dd = dv.D_tensor([D] * C, n_pad, dev)
+    Kd = dv.digits_tensor(locq.T, n_pad, dev)
+    a64 = dv.strengths_tensor(a, "fp64", dev)
+    x = dv.evaluate_cosets_mfma(q, n, b, Mt, Dd, Kd, a64).cpu().numpy()
+    P = D.shape[0]
+    rows = range(P) if P * q ** b * S <= 2e7 else sorted({0, 1, 7, 8, P // 2, P - 2, P - 1})
+    for c in range(C):
+        for p in rows:
+            dig = O.coset_digits(Ms[c], D[p:p + 1], q, b)[0]
+            ref = O.eval_synthetic(dig, locq, a, q)
+            assert rel_err(x[c, p], ref) < 2e-6, (c, p)
+
+
+@pytest.mark.parametrize("q", [4, 3])
+def test_eval_mfma_distinct_delays_per_group(q):
+    """Groups with their own delay rows (B computed per group)."""
+    n, b, S, C, P = 20, 3, 90, 3, 21
+    rng = np.random.default_rng(41)
+    Ms = [rng.integers(0, q, size=(n, b)) for _ in range(C)]
+    Ds = [rng.integers(0, q, size=(P, n)) for _ in range(C)]
+    locq = rng.integers(0, q, size=(n, S))
+    a = rng.uniform(1, 10, S) * np.exp(2j * np.pi * rng.uniform(size=S))
+    dev = dv.current_device()
+    n_pad = dv.n_pad_of(n)
+    x = dv.evaluate_cosets_mfma(q, n, b, dv.Mt_tensor(Ms, b, n_pad, dev), dv.D_tensor(Ds, n_pad, dev),
+                                dv.digits_tensor(locq.T, n_pad, dev), dv.strengths_tensor(a, "fp64", dev))
+    x = x.cpu().numpy()
+    for c in range(C):
+        dig = O.coset_digits(Ms[c], Ds[c], q, b)
+        ref = O.eval_synthetic(dig.reshape(-1, n), locq, a, q).reshape(P, q ** b)
+        assert rel_err(x[c], ref) < 2e-6, c
+
+
 @pytest.mark.parametrize("q,n,b,C,P", [(4, 50, 6, 3, 102), (20, 16, 3, 2, 17), (3, 7, 4, 2, 8), (5, 130, 2, 1, 3)])
 def test_query_digits_bit_exact(q, n, b, C, P):
     print(f"query_digits q={q} n={n} b={b}", flush=True)
@@ -130,13 +175,15 @@ def test_assemble_gather_and_noise_array():
     gsel, rsel = [2, 0], [1, 0]
     for prec, dt in (("fp64", torch.complex128), ("fp32", torch.complex64)):
         s = torch.from_numpy(src).to(dt).to(dev)
-        U = torch.empty((2, 2 * p1, Q), dtype=torch.complex128, device=dev)
-        K.assemble([dict(Q=Q, C=2, R=2, p1=p1, src=s, src_rows=Rsrc * p1, gsel=gsel, rsel=rsel, U=U,
-                         noise=torch.from_numpy(noise).to(dev))])
-        got = U.cpu().numpy()
         sref = s.to(torch.complex128).cpu().numpy()
         ref = np.stack([np.vstack([sref[g, r * p1:(r + 1) * p1] for r in rsel]) for g in gsel]) + noise
-        np.testing.assert_array_equal(got, ref)
+        for bm in (False, True):
+            shape = (2, Q, 2 * p1) if bm else (2, 2 * p1, Q)
+            U = torch.empty(shape, dtype=torch.complex128, device=dev)
+            K.assemble([dict(Q=Q, C=2, R=2, p1=p1, src=s, src_rows=Rsrc * p1, gsel=gsel, rsel=rsel, U=U,
+                             noise=torch.from_numpy(noise).to(dev), bin_major=bm)])
+            got = U.cpu().numpy()
+            np.testing.assert_array_equal(got.transpose(0, 2, 1) if bm else got, ref)
 
 
 def test_assemble_philox_noise_statistics():

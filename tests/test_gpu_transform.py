@@ -29,14 +29,15 @@ def device_peel(Ms, Ds, Us, q, n, cutoff, method="nso", host_loop=False):
     C, P, Q = np.asarray(Us).shape
     b = Ms[0].shape[1]
     n_pad = dv.n_pad_of(n)
-    U = torch.from_numpy(np.ascontiguousarray(np.asarray(Us, dtype=np.complex128))).to(dev)
+    Ubm = np.ascontiguousarray(np.asarray(Us, dtype=np.complex128).transpose(0, 2, 1))     # bin-major
+    U = torch.from_numpy(Ubm).to(dev)
     item = dict(q=q, n=n, b=b, C=C, P=P, p1=n + 1,
                 method=L.QSFT_DETECT_NSO1 if method == "nso" else L.QSFT_DETECT_NOISELESS,
                 cutoff=cutoff, Mt=dv.Mt_tensor(Ms, b, n_pad, dev), D=dv.D_tensor(Ds, n_pad, dev), U=U)
     batch = K.PeelBatch([item])
     batch.run(host_loop=host_loop)
     k, v, _, info = batch.results()[0]
-    return {tuple(r): x for r, x in zip(k, v)}, info, U.cpu().numpy()
+    return {tuple(r): x for r, x in zip(k, v)}, info, U.cpu().numpy().transpose(0, 2, 1)
 
 
 def assert_same_result(got, ref, rtol=1e-9):
