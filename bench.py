@@ -50,12 +50,29 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-workers", type=int, default=16)
     ap.add_argument("--quiet", action="store_true")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="first run one device-to-device copy of this many MiB (a known-size reference for "
+                         "hardware byte counters)")
     return ap.parse_args()
 
 
 def log(msg, quiet=False):
     if not quiet:
         print(msg, file=sys.stderr, flush=True)
+
+
+def measured_traffic(batch):
+    """HBM bytes per k_eval_mfma launch at this batch size, from the committed
+    rocprofv3 counter summary (FETCH_SIZE and WRITE_SIZE per dispatch, each
+    scaled by what the counter reads for a copy of known size); None if absent."""
+    path = os.path.join(ROOT, "profiles", "k_eval_mfma_k1_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            if line.startswith("bytes_per_transform"):
+                return float(line.split("=")[1]) * batch
+    return None
 
 
 def cpu_baseline(seed, workers):
@@ -95,6 +112,15 @@ def main():
 
     from qsft_amd import engine as E
 
+    if args.copy_calibration_mib > 0:
+        nb = args.copy_calibration_mib << 20
+        src = torch.ones(nb, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"calibration copy: {nb} bytes read, {nb} bytes written", args.quiet)
+        del src, dst
+
     q, n, S, b, C, R, snr = (K1[k] for k in ("q", "n", "S", "b", "C", "R", "snr_db"))
     rng = np.random.default_rng(args.seed)
     log(f"generating 2 x {args.batch} synthetic transforms", args.quiet)
@@ -116,19 +142,25 @@ def main():
     lat = []
     launched = {}
     last = {}
+    step_events = []          # HIP events around every stage of every timed step (kernel stream)
+
+    def launch(s):
+        ev = {}
+        engines[s % 2].launch(seed_offset=1000 + s, events=ev)
+        launched[s] = time.perf_counter()
+        step_events.append(ev)
+
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for s in range(min(2, args.steps)):
-        engines[s % 2].launch(seed_offset=1000 + s)
-        launched[s] = time.perf_counter()
+        launch(s)
     for s in range(args.steps):
         e = engines[s % 2]
         packed = e.fetch(copy_stream)
         lat.append(time.perf_counter() - launched[s])
         last[s % 2] = packed
         if s + 2 < args.steps:
-            e.launch(seed_offset=1000 + s + 2)
-            launched[s + 2] = time.perf_counter()
+            launch(s + 2)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     value = args.batch * args.steps / elapsed
@@ -138,17 +170,14 @@ def main():
         res = engines[slot].unpack(packed)
         nm += [E.nmse(k, v, sp) for (k, v, _, _), sp in zip(res, spec_sets[slot])]
 
-    # stage timing with HIP events (outside the timed steps)
-    ev = {}
-    eng.launch(seed_offset=99, events=ev)
-    eng.results()
-    torch.cuda.synchronize()
-    names = ["eval", "dft", "assemble", "peel"]
-    stage_ms = {}
-    prev = ev["start"][0]
-    for nm_ in names:
-        stage_ms[nm_] = prev.elapsed_time(ev[nm_][0])
-        prev = ev[nm_][0]
+    # per-stage kernel time, averaged over the timed steps
+    names = [x for x in ("eval_prep", "eval", "dft", "assemble", "peel") if x in step_events[0]]
+    stage_ms = {x: 0.0 for x in names}
+    for ev in step_events:
+        prev = ev["start"][0]
+        for x in names:
+            stage_ms[x] += prev.elapsed_time(ev[x][0]) / len(step_events)
+            prev = ev[x][0]
     eval_s = stage_ms["eval"] / 1e3
     samples = C * (R * (n + 1)) * q ** b                                # B = C * P * Q per transform
     if eng.eval_mode == "mfma":
@@ -161,7 +190,7 @@ def main():
         mfma_bf16 = 3 * 2.0 * C * q ** b * (32 * nsb) * (16 * npt) * args.batch
         roof = {"kernel": "k_eval_mfma (int8 MFMA phase + bf16x3 MFMA complex GEMM, FP32 accumulation)",
                 "bound": "mfma", "achieved": achieved, "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s",
-                "frac": achieved / PEAK_BF16_TFLOPS, "traffic": None,
+                "frac": achieved / PEAK_BF16_TFLOPS, "traffic": measured_traffic(args.batch),
                 "algorithmic_flops_per_launch": flops, "launch_ms": stage_ms["eval"],
                 "mfma_bf16_executed_tflops": mfma_bf16 / eval_s / 1e12}
     else:

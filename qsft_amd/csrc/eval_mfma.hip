@@ -158,8 +158,8 @@ __global__ __launch_bounds__(256) void k_eval_mfma_prep(const qsft_eval_mfma_des
 // MODE 0: q in {2, 4} (w^e exact in bf16, q a power of two)
 // MODE 1: other powers of two (3-limb W)   MODE 2: any other q (3-limb W)
 template <int MODE, int NKB>
-__global__ __launch_bounds__(kMfThreads) void k_eval_mfma(const qsft_eval_mfma_desc* __restrict__ descs,
-                                                          const int32_t* __restrict__ pre, int n_inst) {
+__device__ __forceinline__ void eval_mfma_body(const qsft_eval_mfma_desc* __restrict__ descs,
+                                               const int32_t* __restrict__ pre, int n_inst) {
     constexpr int WL = (MODE == 0) ? 1 : 3;
     __shared__ __align__(16) v4u bsh[kStage];                     // 24 KiB of B fragments
     __shared__ uint32_t wtab[3][QSFT_MAX_Q];                      // (Re | Im << 16) bf16 bits per limb
@@ -241,7 +241,8 @@ __global__ __launch_bounds__(kMfThreads) void k_eval_mfma(const qsft_eval_mfma_d
         }
     };
 
-    // prologue: support block 0 into registers, then into LDS
+    // prologue: support block 0 into registers, then into LDS. p-tiles beyond
+    // npt_here are staged as zeros so the MFMA loop below has no branches.
     v4u stage[kStagePerThread];
     v4i kf[NKB];
     {
@@ -249,14 +250,11 @@ __global__ __launch_bounds__(kMfThreads) void k_eval_mfma(const qsft_eval_mfma_d
 #pragma unroll
         for (int i = 0; i < kStagePerThread; ++i) {
             const int e = tid + i * kMfThreads;
-            if (e < nstage) stage[i] = Bg[fb + e];
+            stage[i] = (e < nstage) ? Bg[fb + e] : v4u{0u, 0u, 0u, 0u};
         }
         load_k(0, kf);
 #pragma unroll
-        for (int i = 0; i < kStagePerThread; ++i) {
-            const int e = tid + i * kMfThreads;
-            if (e < nstage) bsh[e] = stage[i];
-        }
+        for (int i = 0; i < kStagePerThread; ++i) bsh[tid + i * kMfThreads] = stage[i];
     }
     __syncthreads();     // fragments of block 0 and wtab visible
 
@@ -299,34 +297,30 @@ __global__ __launch_bounds__(kMfThreads) void k_eval_mfma(const qsft_eval_mfma_d
 #pragma unroll
             for (int i = 0; i < kStagePerThread; ++i) {
                 const int e = tid + i * kMfThreads;
-                if (e < nstage) stage[i] = Bg[fb + e];
+                stage[i] = (e < nstage) ? Bg[fb + e] : v4u{0u, 0u, 0u, 0u};
             }
             load_k(sb + 1, kf);
         }
+        // limb-major order: an accumulator is revisited only every 8 x 4 MFMAs
 #pragma unroll
-        for (int pt = 0; pt < kPtPerBlock; ++pt) {
-            if (pt < npt_here) {
+        for (int wl = 0; wl < WL; ++wl) {
 #pragma unroll
-                for (int bl = 0; bl < kLimbs; ++bl) {
+            for (int bl = 0; bl < kLimbs; ++bl) {
+                if (wl + bl > 2) continue;                  // drop limb products below 2^-24
+#pragma unroll
+                for (int pt = 0; pt < kPtPerBlock; ++pt) {
                     const v8bf bf = __builtin_bit_cast(v8bf, bsh[(pt * kLimbs + bl) * 64 + lane]);
 #pragma unroll
-                    for (int wl = 0; wl < WL; ++wl) {
-                        if (wl + bl > 2) continue;          // drop limb products below 2^-24
-#pragma unroll
-                        for (int t = 0; t < 4; ++t)
-                            acc[t][pt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(v8bf, wa[t][wl]), bf,
-                                                                               acc[t][pt], 0, 0, 0);
-                    }
+                    for (int t = 0; t < 4; ++t)
+                        acc[t][pt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(v8bf, wa[t][wl]), bf,
+                                                                           acc[t][pt], 0, 0, 0);
                 }
             }
         }
         __syncthreads();     // every wave is done with this block's fragments
         if (more) {
 #pragma unroll
-            for (int i = 0; i < kStagePerThread; ++i) {
-                const int e = tid + i * kMfThreads;
-                if (e < nstage) bsh[e] = stage[i];
-            }
+            for (int i = 0; i < kStagePerThread; ++i) bsh[tid + i * kMfThreads] = stage[i];
         }
         __syncthreads();     // next block's fragments visible
     }
@@ -359,6 +353,23 @@ __global__ __launch_bounds__(kMfThreads) void k_eval_mfma(const qsft_eval_mfma_d
             }
         }
     }
+}
+
+// Default entry: one 256-thread block per CU slot, the register budget the body
+// needs without spilling for every (MODE, NKB).
+template <int MODE, int NKB>
+__global__ __launch_bounds__(kMfThreads) void k_eval_mfma(const qsft_eval_mfma_desc* __restrict__ descs,
+                                                          const int32_t* __restrict__ pre, int n_inst) {
+    eval_mfma_body<MODE, NKB>(descs, pre, n_inst);
+}
+
+// Two waves per SIMD: caps the body at 256 VGPR+AGPR. Only the one-limb phase
+// (q in {2, 4}) with one k-block (n <= 64) fits that without scratch; it then
+// hides the LDS and global latency of one block behind the MFMAs of the other.
+template <int MODE, int NKB>
+__global__ __launch_bounds__(kMfThreads) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_eval_mfma_occ2(
+    const qsft_eval_mfma_desc* __restrict__ descs, const int32_t* __restrict__ pre, int n_inst) {
+    eval_mfma_body<MODE, NKB>(descs, pre, n_inst);
 }
 
 }  // namespace qsft
@@ -396,7 +407,12 @@ template <int MODE>
 static hipError_t launch_mfma(int nkb, const qsft_eval_mfma_desc* d, const int32_t* pre, int n, int blocks,
                               hipStream_t st) {
     switch (nkb) {
-        case 1: hipLaunchKernelGGL((k_eval_mfma<MODE, 1>), dim3(blocks), dim3(kMfThreads), 0, st, d, pre, n); break;
+        case 1:
+            if constexpr (MODE == 0)
+                hipLaunchKernelGGL((k_eval_mfma_occ2<MODE, 1>), dim3(blocks), dim3(kMfThreads), 0, st, d, pre, n);
+            else
+                hipLaunchKernelGGL((k_eval_mfma<MODE, 1>), dim3(blocks), dim3(kMfThreads), 0, st, d, pre, n);
+            break;
         case 2: hipLaunchKernelGGL((k_eval_mfma<MODE, 2>), dim3(blocks), dim3(kMfThreads), 0, st, d, pre, n); break;
         case 3: hipLaunchKernelGGL((k_eval_mfma<MODE, 3>), dim3(blocks), dim3(kMfThreads), 0, st, d, pre, n); break;
         default: hipLaunchKernelGGL((k_eval_mfma<MODE, 4>), dim3(blocks), dim3(kMfThreads), 0, st, d, pre, n); break;

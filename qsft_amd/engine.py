@@ -149,7 +149,10 @@ class Batch:
                 e.record()
                 events.setdefault(name, []).append(e)
         mark("start")
-        self.eval.launch()
+        if events is not None and hasattr(self.eval, "launch_split"):
+            self.eval.launch_split(lambda: mark("eval_prep"))
+        else:
+            self.eval.launch()
         mark("eval")
         self.dft.launch()
         mark("dft")

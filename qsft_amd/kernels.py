@@ -159,6 +159,16 @@ class EvalMfma:
             L.check(lib.qsft_eval_mfma(_vp(dptr), _vp(pp), _vp(pm), n, tp, tm, word, stream_handle()),
                     "qsft_eval_mfma")
 
+    def launch_split(self, between):
+        """The B-fragment prep launches, then `between()`, then the GEMM
+        launches (so events can time the GEMM kernel on its own)."""
+        st = stream_handle()
+        for _, dptr, pp, pm, n, tp, tm, word in self.launches:
+            L.check(lib.qsft_eval_mfma(_vp(dptr), _vp(pp), _vp(pm), n, tp, 0, word, st), "qsft_eval_mfma(prep)")
+        between()
+        for _, dptr, pp, pm, n, tp, tm, word in self.launches:
+            L.check(lib.qsft_eval_mfma(_vp(dptr), _vp(pp), _vp(pm), n, 0, tm, word, st), "qsft_eval_mfma(main)")
+
 
 class QueryDigits:
     """items: dicts q, n, b, C, P, Mt, D, m (int8 [C][P][q^b][n_pad] out)."""
