@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define QSFT_ABI_VERSION 1
+#define QSFT_ABI_VERSION 2
 #define QSFT_MAX_Q 64          /* alphabet size limit (digits are int8, w-tables live in LDS) */
 #define QSFT_MAX_N 256         /* digit count limit (n_pad <= 256)                              */
 #define QSFT_MAX_B 16          /* sub-sampling dimension limit                                  */
@@ -200,13 +200,15 @@ int qsft_assemble(const qsft_assemble_desc* dev_descs, const int32_t* dev_pre, i
  * Replaces the while-loop of QSFT.transform (qsft/qsft.py:142-255) with
  * singleton_detection (qsft/reconstruct.py:12-31, 100-113, 132-167).
  * U is bin-major: [C][Q][P] complex FP64, one bin's P rows contiguous.
- * Per iteration: kernel A (one wave per bin) scans the frozen U -- every bin
- * on the first pass, afterwards only bins that were non-zeroton on the
+ * A screening kernel stamps the non-zeroton bins for the first pass. Per
+ * iteration: kernel A (one wave per 64 consecutive bins) scans the frozen U --
+ * the stamped bins: on later passes those that were non-zeroton on the
  * previous pass or were peeled into (an unchanged zeroton stays a zeroton) --
- * classifies zeroton / singleton / multiton (k, rho, bin match, residual) and
- * registers each singleton in a device hash table keyed by k; kernel B (one
- * wave per singleton) peels each distinct k with the rho of its last
- * (group, bin) from its bin in every group. The stop rule (no singletons or
+ * classifies zeroton / singleton / multiton (k, rho, bin match,
+ * residual) and registers the wave's singletons in a device hash table keyed
+ * by k, one lane per singleton; kernel B (same bin partition) peels each
+ * distinct k with the rho of its last (group, bin) from its bin in every
+ * group. The stop rule (no singletons or
  * no multitons, the pass still peels), the q^n ball cap and the iteration
  * cap run on the device: no host round trip. The final kernel forms the
  * append-order running mean of every detection of each k
@@ -220,7 +222,7 @@ typedef struct qsft_peel_desc {
     int32_t max_iter;       /* 15 in the reference (at most 15)                             */
     int32_t hcap_log2;      /* log2 of hash slots H                                         */
     int32_t out_cap;        /* capacity of the output arrays                                */
-    int32_t _pad0;
+    int32_t flags;          /* QSFT_PEEL_* bits, set by the caller after qsft_peel_bind()   */
     double cutoff;          /* threshold per row: |col|^2 > cutoff * P                      */
     double peel_cap;        /* q^n (as double, saturating): cap on peeled balls            */
     const int8_t* Mt;       /* [C][b][n_pad]                                                */
@@ -230,10 +232,8 @@ typedef struct qsft_peel_desc {
     int8_t* kbin;           /* [C*Q][n_pad]  k of singleton bins of the current pass        */
     double* rho_bin;        /* [C*Q][2]                                                     */
     int32_t* slot_bin;      /* [C*Q]         hash slot of a singleton bin, -1 otherwise     */
-    int32_t* mark;          /* [C*Q]         last pass + 1 whose scan list holds the bin    */
-    int32_t* list0;         /* [C*Q]         scan lists, alternating between passes         */
-    int32_t* list1;         /* [C*Q]                                                        */
-    int32_t* slist;         /* [C*Q]         singleton bins of the current pass             */
+    int32_t* mark;          /* [C*Q]         pass that scans the bin next (-1: none)         */
+    int32_t* sstamp;        /* [C*Q]         pass + 1 in which the bin was a singleton      */
     uint64_t* tag;          /* [H] 0 = empty                                                */
     uint64_t* owner;        /* [H] first (iteration, bin) of the key                        */
     uint64_t* last;         /* [H] last (iteration, bin) of the key                         */
@@ -250,9 +250,12 @@ typedef struct qsft_peel_desc {
     uint64_t* out_order;    /* [out_cap]        first (iteration, bin) of k: dict order     */
 } qsft_peel_desc;
 
+/* qsft_peel_desc::flags */
+#define QSFT_PEEL_SHARED_D 1    /* every group uses the delay rows of group 0            */
+
 /* counter layout (uint32 words); per iteration i at 8 i + :
  *   0 singletons, 1 multitons, 2 distinct balls, 3 active flag,
- *   4 bins in the scan list of pass i + 1, 5 singleton bins of pass i */
+ *   4 bins scanned in pass i */
 #define QSFT_CTR_ITER_STRIDE 8
 #define QSFT_CTR_LOG 128        /* records appended                                    */
 #define QSFT_CTR_OUT 129        /* outputs written                                     */
@@ -274,9 +277,11 @@ int qsft_peel_slot_blocks(const qsft_peel_desc* host_desc);  /* blocks of the pe
 int qsft_peel_run(const qsft_peel_desc* dev_descs, const int32_t* dev_pre_bins,
                   const int32_t* dev_pre_slots, int n_inst, int blocks_bins, int blocks_slots,
                   int max_iter, void* stream);
-/* The same steps one at a time, for a host-driven loop (the plain path). */
-int qsft_peel_init(const qsft_peel_desc* dev_descs, const int32_t* dev_pre_slots, int n_inst,
-                   int blocks_slots, void* stream);
+/* The same steps one at a time, for a host-driven loop (the plain path).
+ * qsft_peel_init clears the tables and screens every bin of the first pass
+ * (|col|^2 > cutoff * P): only non-zeroton bins are stamped for pass 0. */
+int qsft_peel_init(const qsft_peel_desc* dev_descs, const int32_t* dev_pre_bins, const int32_t* dev_pre_slots,
+                   int n_inst, int blocks_bins, int blocks_slots, void* stream);
 int qsft_peel_iteration(const qsft_peel_desc* dev_descs, const int32_t* dev_pre_bins, int n_inst,
                         int blocks_bins, int iteration, void* stream);
 int qsft_peel_finalize(const qsft_peel_desc* dev_descs, const int32_t* dev_pre_slots, int n_inst,

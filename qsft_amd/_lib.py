@@ -22,6 +22,7 @@ QSFT_NOISE_NONE, QSFT_NOISE_ARRAY, QSFT_NOISE_PHILOX = 0, 1, 2
 QSFT_DEVERR_TABLE_FULL, QSFT_DEVERR_TAG_COLLISION, QSFT_DEVERR_OUT_FULL = 1, 2, 4
 QSFT_MAX_Q, QSFT_MAX_N, QSFT_MAX_B, QSFT_MAX_C, QSFT_MAX_R = 64, 256, 16, 16, 16
 QSFT_DFT_MAX_PASSES = 4
+QSFT_PEEL_SHARED_D = 1
 QSFT_CTR_ITER_STRIDE = 8
 QSFT_CTR_LOG, QSFT_CTR_OUT, QSFT_CTR_ERROR, QSFT_CTR_ITERS, QSFT_CTR_PEELED = 128, 129, 130, 131, 132
 QSFT_CTR_WORDS = 136
@@ -57,11 +58,11 @@ class AssembleDesc(ctypes.Structure):
 class PeelDesc(ctypes.Structure):
     _fields_ = [("q", i32), ("n", i32), ("b", i32), ("C", i32), ("P", i32), ("Q", i32),
                 ("n_pad", i32), ("p1", i32), ("method", i32), ("max_iter", i32),
-                ("hcap_log2", i32), ("out_cap", i32), ("_pad0", i32),
+                ("hcap_log2", i32), ("out_cap", i32), ("flags", i32),
                 ("cutoff", f64), ("peel_cap", f64),
                 ("Mt", vp), ("D", vp), ("U", vp),
-                ("kbin", vp), ("rho_bin", vp), ("slot_bin", vp), ("mark", vp), ("list0", vp), ("list1", vp),
-                ("slist", vp), ("tag", vp), ("owner", vp),
+                ("kbin", vp), ("rho_bin", vp), ("slot_bin", vp), ("mark", vp), ("sstamp", vp),
+                ("tag", vp), ("owner", vp),
                 ("last", vp), ("head", vp), ("count", vp), ("key", vp), ("rec_time", vp),
                 ("rec_rho", vp), ("rec_next", vp), ("ctr", vp),
                 ("out_k", vp), ("out_val", vp), ("out_order", vp)]
@@ -104,7 +105,7 @@ SIGNATURES = {
     "qsft_peel_bin_blocks": (ctypes.c_int, [ctypes.POINTER(PeelDesc)]),
     "qsft_peel_slot_blocks": (ctypes.c_int, [ctypes.POINTER(PeelDesc)]),
     "qsft_peel_run": (ctypes.c_int, [vp, vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp]),
-    "qsft_peel_init": (ctypes.c_int, [vp, vp, ctypes.c_int, ctypes.c_int, vp]),
+    "qsft_peel_init": (ctypes.c_int, [vp, vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp]),
     "qsft_peel_iteration": (ctypes.c_int, [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp]),
     "qsft_peel_finalize": (ctypes.c_int, [vp, vp, ctypes.c_int, ctypes.c_int, vp]),
     "qsft_peel_pack_header_bytes": (ctypes.c_size_t, [ctypes.c_int]),

@@ -3,58 +3,59 @@
 // add transform-domain noise. Logical element (c, p = r*p1 + i, j) is
 //   src[gsel[c]][rsel[r]*p1 + i][j] + noise(c, p, j)
 // and is stored row-major [C][R*p1][Q] or bin-major [C][Q][R*p1] (the
-// peeler's layout). The bin-major store goes through a 32-row x 64-bin LDS
-// tile so both the reads (rows of the source) and the writes (columns of U)
-// are contiguous. HBM-bound: 8 or 16 B read + 16 B write (+16 B noise read)
-// per element. Philox noise is indexed by the logical element, so it does
-// not depend on the layout.
+// peeler's layout). HBM-bound: 8 or 16 B read + 16 B write (+16 B noise read
+// in the array mode) per element.
+//
+// Bin-major: a workgroup owns 16 bins x up to 128 rows; it reads the rows'
+// 16-bin segments into an LDS tile, then writes the 16 bins' columns, which
+// are one contiguous run of U. Philox noise is drawn in the write phase, one
+// Philox block for the two elements (c, p, 2h) and (c, p, 2h + 1), so a
+// thread writes two adjacent bins of one row. The noise is a function of the
+// logical element only, so both layouts draw the same values.
 #include "qsft_common.h"
 
 namespace qsft {
 
 constexpr int kAsmThreads = 256;
 constexpr int kAsmPerThread = 4;
-constexpr int kTileP = 32;
-constexpr int kTileJ = 64;
+constexpr int kBmBins = 16;        // bins per bin-major tile
+constexpr int kBmRows = 128;       // rows per bin-major tile (row chunks beyond)
 
-__device__ __forceinline__ double2 asm_value(const qsft_assemble_desc& d, int c, int p, int64_t j,
-                                             uint64_t seed_offset) {
-    const int Q = d.Q, rows = d.R * d.p1;
+// Counter of the Philox block shared by bins 2h and 2h + 1 of row (c, p).
+__device__ __forceinline__ uint64_t noise_counter(int c, int p, int rows, int64_t Q, int64_t j) {
+    return ((uint64_t)c * rows + p) * (uint64_t)((Q + 1) >> 1) + (uint64_t)(j >> 1);
+}
+
+__device__ __forceinline__ double2 src_value(const qsft_assemble_desc& d, int c, int p, int64_t j) {
     const int r = p / d.p1, i = p - r * d.p1;
-    const int64_t src_idx = ((int64_t)d.gsel[c] * d.src_rows + (int64_t)d.rsel[r] * d.p1 + i) * Q + j;
-    double vx, vy;
+    const int64_t src_idx = ((int64_t)d.gsel[c] * d.src_rows + (int64_t)d.rsel[r] * d.p1 + i) * d.Q + j;
     if (d.src_prec == QSFT_F32) {
         const float2 v = reinterpret_cast<const float2*>(d.src)[src_idx];
-        vx = v.x;
-        vy = v.y;
-    } else {
-        const double2 v = reinterpret_cast<const double2*>(d.src)[src_idx];
-        vx = v.x;
-        vy = v.y;
+        return make_double2(v.x, v.y);
     }
-    const int64_t e = ((int64_t)c * rows + p) * Q + j;             // logical element index
+    return reinterpret_cast<const double2*>(d.src)[src_idx];
+}
+
+__device__ __forceinline__ double2 add_array_noise(const qsft_assemble_desc& d, double2 v, int64_t e) {
     if (d.noise_mode == QSFT_NOISE_ARRAY) {
         const double2 z = reinterpret_cast<const double2*>(d.noise)[e];
-        vx = vx + z.x;
-        vy = vy + z.y;
-    } else if (d.noise_mode == QSFT_NOISE_PHILOX) {
-        const double comp_sd = d.sigma * 0.70710678118654752440;  // per component: sqrt(sigma^2 / 2)
-        const double2 z = normal2((uint64_t)e, d.seed + seed_offset);
-        vx = vx + comp_sd * z.x;
-        vy = vy + comp_sd * z.y;
+        v.x = v.x + z.x;
+        v.y = v.y + z.y;
     }
-    return make_double2(vx, vy);
+    return v;
 }
 
 __global__ __launch_bounds__(kAsmThreads) void k_assemble(const qsft_assemble_desc* __restrict__ descs,
                                                           const int32_t* __restrict__ pre, int n_inst,
                                                           uint64_t seed_offset) {
-    __shared__ double2 tile[kTileP][kTileJ + 1];
+    __shared__ double2 tile[kBmRows][kBmBins + 1];
     const int inst = find_inst(pre, n_inst, blockIdx.x);
     const qsft_assemble_desc& d = descs[inst];
-    const int Q = d.Q;
+    const int64_t Q = d.Q;
     const int rows = d.R * d.p1;
     const int local = blockIdx.x - pre[inst];
+    const double comp_sd = d.sigma * 0.70710678118654752440;      // per component: sqrt(sigma^2 / 2)
+    const uint64_t seed = d.seed + seed_offset;
     if (!d.bin_major) {
         const int64_t total = (int64_t)d.C * rows * Q;
         const int64_t e0 = (int64_t)local * (kAsmThreads * kAsmPerThread) + threadIdx.x;
@@ -65,25 +66,56 @@ __global__ __launch_bounds__(kAsmThreads) void k_assemble(const qsft_assemble_de
             const int64_t rowg = e / Q;
             const int c = (int)(rowg / rows);
             const int p = (int)(rowg - (int64_t)c * rows);
-            reinterpret_cast<double2*>(d.U)[e] = asm_value(d, c, p, j, seed_offset);
+            double2 v = add_array_noise(d, src_value(d, c, p, j), e);
+            if (d.noise_mode == QSFT_NOISE_PHILOX) {
+                const float4 z = normal4(noise_counter(c, p, rows, Q, j), seed);
+                v.x = v.x + comp_sd * (double)((j & 1) ? z.z : z.x);
+                v.y = v.y + comp_sd * (double)((j & 1) ? z.w : z.y);
+            }
+            reinterpret_cast<double2*>(d.U)[e] = v;
         }
         return;
     }
-    const int nj = (Q + kTileJ - 1) / kTileJ, np = (rows + kTileP - 1) / kTileP;
-    const int c = local / (nj * np);
-    const int rem = local - c * nj * np;
-    const int jt = rem / np, pt = rem - jt * np;
-    for (int e = threadIdx.x; e < kTileP * kTileJ; e += kAsmThreads) {
-        const int pr = e / kTileJ, jj = e - pr * kTileJ;
-        const int p = pt * kTileP + pr, j = jt * kTileJ + jj;
-        if (p < rows && j < Q) tile[pr][jj] = asm_value(d, c, p, j, seed_offset);
+    const int nj = (int)((Q + kBmBins - 1) / kBmBins), npc = (rows + kBmRows - 1) / kBmRows;
+    const int c = local / (nj * npc);
+    const int rem = local - c * nj * npc;
+    const int jt = rem / npc, pc = rem - jt * npc;
+    const int p0 = pc * kBmRows, Pc = min(kBmRows, rows - p0);
+    const int64_t j0 = (int64_t)jt * kBmBins;
+    const int Jc = (int)min((int64_t)kBmBins, Q - j0);
+    // read: 16 lanes cover one row's 16-bin segment
+    for (int e = threadIdx.x; e < Pc * kBmBins; e += kAsmThreads) {
+        const int pr = e >> 4, jj = e & 15;
+        if (jj < Jc) {
+            const int p = p0 + pr;
+            const int64_t j = j0 + jj;
+            tile[pr][jj] = add_array_noise(d, src_value(d, c, p, j), ((int64_t)c * rows + p) * Q + j);
+        }
     }
     __syncthreads();
+    // write: element pairs (2h, 2h + 1) of one row; consecutive lanes take consecutive rows
     double2* __restrict__ U = reinterpret_cast<double2*>(d.U);
-    for (int e = threadIdx.x; e < kTileP * kTileJ; e += kAsmThreads) {
-        const int jj = e / kTileP, pr = e - jj * kTileP;
-        const int p = pt * kTileP + pr, j = jt * kTileJ + jj;
-        if (p < rows && j < Q) U[((int64_t)c * Q + j) * rows + p] = tile[pr][jj];
+    const int npairs = (Jc + 1) >> 1;
+    // o / Pc by multiply-high: exact for o < 1024, Pc in [2, 128] (checked exhaustively); Pc = 1 is o itself
+    const uint32_t mP = (uint32_t)(0xFFFFFFFFull / (uint32_t)Pc) + 1u;
+    const bool philox = d.noise_mode == QSFT_NOISE_PHILOX;
+    for (int o = threadIdx.x; o < npairs * Pc; o += kAsmThreads) {
+        const int h = (Pc == 1) ? o : (int)__umulhi((uint32_t)o, mP);
+        const int pr = o - h * Pc;
+        const int jj = 2 * h;
+        const int p = p0 + pr;
+        const int64_t j = j0 + jj;
+        double2 v0 = tile[pr][jj];
+        double2 v1 = (jj + 1 < Jc) ? tile[pr][jj + 1] : make_double2(0.0, 0.0);
+        if (philox) {
+            const float4 z = normal4(noise_counter(c, p, rows, Q, j), seed);
+            v0.x = v0.x + comp_sd * (double)z.x;
+            v0.y = v0.y + comp_sd * (double)z.y;
+            v1.x = v1.x + comp_sd * (double)z.z;
+            v1.y = v1.y + comp_sd * (double)z.w;
+        }
+        U[((int64_t)c * Q + j) * rows + p] = v0;
+        if (jj + 1 < Jc) U[((int64_t)c * Q + j + 1) * rows + p] = v1;
     }
 }
 
@@ -101,7 +133,7 @@ extern "C" int qsft_assemble_blocks(const qsft_assemble_desc* h) {
     const long long rows = (long long)h->R * h->p1;
     long long blocks;
     if (h->bin_major) {
-        blocks = (long long)h->C * ((h->Q + kTileJ - 1) / kTileJ) * ((rows + kTileP - 1) / kTileP);
+        blocks = (long long)h->C * ((h->Q + kBmBins - 1) / kBmBins) * ((rows + kBmRows - 1) / kBmRows);
     } else {
         const long long total = (long long)h->C * rows * h->Q;
         const long long per = kAsmThreads * kAsmPerThread;

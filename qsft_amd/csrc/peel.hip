@@ -21,27 +21,31 @@
 // xor-butterfly reductions, bit-identical on every lane) and coordinates
 // i = lane + 1 (+ 64) for nso1 (one atan2 per lane).
 //
-// Scan lists: pass 0 scans every bin; a later pass scans only the bins that
-// were non-zeroton on the previous pass or were peeled into by it (an
-// untouched zeroton keeps its column, so it stays a zeroton). A per-bin
-// stamp (atomicMax with pass + 1) keeps each bin in a list once.
+// Scan stamps: a streaming screen stamps the non-zeroton bins for pass 0; a
+// later pass scans only the bins that were non-zeroton on the previous pass or
+// were peeled into by it (an untouched zeroton keeps its column, so it stays
+// a zeroton). Both per-pass kernels mark such a bin with a plain store of
+// pass + 1 (every writer stores the same value), and the next pass's waves
+// test the stamps of their 64 bins with one load and a ballot: no list, no
+// returning atomic per bin.
 //
 // Kernel A (detect) registers each singleton in an open-addressing hash
 // table keyed by a 64-bit hash ("tag") of k: CAS claims a slot, atomicMin /
 // atomicMax on (pass+1) << 32 | bin record the first and the last detection,
-// and a record (time, rho) is pushed on the slot's lock-free list. Kernel B
-// (peel, one wave per singleton) checks that the detection's k equals its
-// slot's key (a 64-bit tag collision sets an error bit instead of merging
-// two keys), writes the key of new slots, and lets the last detection of each
-// key peel it from every group with coalesced FP64 atomics over the
-// contiguous column. No kernel waits on another thread's progress.
+// and a record (time, rho) is pushed on the slot's lock-free list; a wave's
+// singletons are registered together, one lane each. Kernel B (peel) checks
+// that the detection's k equals its slot's key (a 64-bit tag collision sets
+// an error bit instead of merging two keys), writes the key of new slots, and
+// lets the last detection of each key peel it from every group with
+// coalesced FP64 atomics over the contiguous column. No kernel waits on
+// another thread's progress.
 #include "qsft_common.h"
 
 namespace qsft {
 
 constexpr int kWaves = 4;             // waves per block of the per-bin kernels
 constexpr int kBinThreads = 64 * kWaves;
-constexpr int kBinsPerWave = 16;      // first-pass bins per wave (grid sizing)
+constexpr int kBinsPerWave = 64;      // consecutive bins owned by one wave (one per lane) in the per-bin kernels
 constexpr int kRowsPerLane = 4;       // P <= 256
 constexpr int kSlotThreads = 256;
 constexpr uint32_t kNil = 0xFFFFFFFFu;
@@ -101,15 +105,6 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 }
 
-// Put bin t in the scan list of pass it + 1 (once).
-__device__ __forceinline__ void enlist(const qsft_peel_desc& d, int it, int t) {
-    if (atomicMax(&d.mark[t], it + 1) < it + 1) {
-        int32_t* next = ((it + 1) & 1) ? d.list1 : d.list0;
-        const uint32_t pos = atomicAdd(&d.ctr[it * QSFT_CTR_ITER_STRIDE + 4], 1u);
-        next[pos] = t;
-    }
-}
-
 __global__ __launch_bounds__(kSlotThreads) void k_peel_init(const qsft_peel_desc* __restrict__ descs,
                                                             const int32_t* __restrict__ pre, int n_inst) {
     const int inst = find_inst(pre, n_inst, blockIdx.x);
@@ -125,13 +120,94 @@ __global__ __launch_bounds__(kSlotThreads) void k_peel_init(const qsft_peel_desc
         d.head[h] = kNil;
         d.count[h] = 0;
     }
-    if (h < d.C * d.Q) d.mark[h] = 0;
+    if (h < d.C * d.Q) d.sstamp[h] = 0;
 }
 
-__global__ __launch_bounds__(kBinThreads) void k_peel_detect(const qsft_peel_desc* __restrict__ descs,
+__device__ __forceinline__ void load_col(double2 (&u)[kRowsPerLane], const double2* __restrict__ col, int P,
+                                         int lane) {
+#pragma unroll
+    for (int r = 0; r < kRowsPerLane; ++r) {
+        const int p = lane + 64 * r;
+        u[r] = (p < P) ? col[p] : make_double2(0.0, 0.0);
+    }
+}
+
+// q^(b - 1 - dg): weight of digit dg (MSB first) in a bin index
+__device__ __forceinline__ int digit_weight(int q, int b, int dg) {
+    int pw = 1;
+    for (int tt = dg + 1; tt < b; ++tt) pw *= q;
+    return pw;
+}
+
+// |col|^2 as every zeroton test computes it: per-lane sums over the lane's
+// rows in order with explicit FMAs, an xor butterfly, then sqrt and square
+// (np.linalg.norm(col) ** 2). The screen and kernel A share it, so they
+// classify a column identically.
+__device__ __forceinline__ double bin_energy(const double2 (&u)[kRowsPerLane]) {
+    double sre = 0.0, sim = 0.0;
+#pragma unroll
+    for (int r = 0; r < kRowsPerLane; ++r) {
+        sre = __fma_rn(u[r].x, u[r].x, sre);
+        sim = __fma_rn(u[r].y, u[r].y, sim);
+    }
+    sre = wave_sum(sre);
+    sim = wave_sum(sim);
+    const double e = sqrt(__dadd_rn(sre, sim));
+    return __dmul_rn(e, e);
+}
+
+// First-pass screen. Wave w owns bins [64 w, 64 w + 64); lane L stamps bin
+// 64 w + L for pass 0 iff it is not a zeroton, and -1 (never) otherwise. Four
+// columns are loaded before their energies are reduced, so several loads per
+// lane are in flight: this kernel streams U once at HBM speed, and kernel A
+// then visits only the stamped bins.
+__global__ __launch_bounds__(kBinThreads) void k_peel_screen(const qsft_peel_desc* __restrict__ descs,
+                                                             const int32_t* __restrict__ pre, int n_inst) {
+    const int inst = find_inst(pre, n_inst, blockIdx.x);
+    const qsft_peel_desc& d = descs[inst];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int CQ = d.C * d.Q, P = d.P;
+    const int base = ((blockIdx.x - pre[inst]) * kWaves + wave) * kBinsPerWave;
+    if (base >= CQ) return;
+    const int nb = min(kBinsPerWave, CQ - base);
+    const double thr = d.cutoff * (double)P;
+    const double2* __restrict__ Ub = reinterpret_cast<const double2*>(d.U) + (size_t)base * P;
+    bool nz = false;
+    for (int g = 0; g < nb; g += 4) {
+        double2 u[4][kRowsPerLane];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (g + k < nb) {
+                load_col(u[k], Ub + (size_t)(g + k) * P, P, lane);
+            } else {
+#pragma unroll
+                for (int r = 0; r < kRowsPerLane; ++r) u[k][r] = make_double2(0.0, 0.0);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const double e = bin_energy(u[k]);
+            if (lane == g + k) nz = e > thr;
+        }
+    }
+    if (base + lane < CQ) d.mark[base + lane] = nz ? 0 : -1;
+}
+
+// Kernel A. Wave w of an instance owns bins [64 w, 64 w + 64): it classifies
+// the ones this pass scans (mark == pass), prefetching
+// the next bin's column while it works on the current one, and buffers its
+// singletons in LDS. Then one lane per buffered singleton registers it in the
+// hash table, so the atomics of up to 16 registrations are in flight
+// together, and the pass counters take one atomic per wave. Three waves per
+// SIMD fit without scratch (the FP64 atan2 is the largest register user).
+__global__ __launch_bounds__(kBinThreads) __attribute__((amdgpu_waves_per_eu(3))) void k_peel_detect(const qsft_peel_desc* __restrict__ descs,
                                                              const int32_t* __restrict__ pre, int n_inst, int it) {
     __shared__ double2 tab[QSFT_MAX_Q];
     __shared__ uint32_t kws[kWaves][QSFT_MAX_N / 4];
+    __shared__ double2 colw[kWaves][64 * kRowsPerLane];
+    __shared__ uint64_t sb_h[kWaves][kBinsPerWave];
+    __shared__ double2 sb_rho[kWaves][kBinsPerWave];
+    __shared__ int32_t sb_t[kWaves][kBinsPerWave];
     const int inst = find_inst(pre, n_inst, blockIdx.x);
     const qsft_peel_desc& d = descs[inst];
     uint32_t* __restrict__ ctr = d.ctr;
@@ -143,169 +219,215 @@ __global__ __launch_bounds__(kBinThreads) void k_peel_detect(const qsft_peel_des
     __syncthreads();
 
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int CQ = C * Q;
+    const int base = ((blockIdx.x - pre[inst]) * kWaves + wave) * kBinsPerWave;
+    if (base >= CQ) return;
+    const bool want = base + lane < CQ && d.mark[base + lane] == it;
+    uint64_t todo = __ballot(want);
+    if (todo == 0) return;
+    const uint32_t scanned = (uint32_t)__popcll(todo);
+
     uint32_t* kk = kws[wave];
     int8_t* kb8 = reinterpret_cast<int8_t*>(kk);
-    const int CQ = C * Q;
-    const int count = (it == 0) ? CQ : (int)ctr[(it - 1) * QSFT_CTR_ITER_STRIDE + 4];
-    const int32_t* __restrict__ list = (it == 0) ? nullptr : ((it & 1) ? d.list1 : d.list0);
-    const int waves_inst = (pre[inst + 1] - pre[inst]) * kWaves;
+    double2* cw = colw[wave];
     const ModQ mq((uint32_t)q);
     const double thr = d.cutoff * (double)P;
     const double twopi = 6.283185307179586;
+    const double2* __restrict__ Ub = reinterpret_cast<const double2*>(d.U);
+    int nz = 0, ns = 0;                                            // wave-uniform
 
-    for (int e = (blockIdx.x - pre[inst]) * kWaves + wave; e < count; e += waves_inst) {
-        const int t = list ? list[e] : e;
-        const int c = t / Q, j = t - c * Q;
-        const double2* __restrict__ col = reinterpret_cast<const double2*>(d.U) + (size_t)t * P;
+    int cur = __builtin_ctzll(todo);
+    double2 un[kRowsPerLane];
+    load_col(un, Ub + (size_t)(base + cur) * P, P, lane);
+    for (;;) {
+        const int t = base + cur;
         double2 u[kRowsPerLane];
-        double sre = 0.0, sim = 0.0;
 #pragma unroll
-        for (int r = 0; r < kRowsPerLane; ++r) {
-            const int p = lane + 64 * r;
-            u[r] = (p < P) ? col[p] : make_double2(0.0, 0.0);
-            sre += u[r].x * u[r].x;
-            sim += u[r].y * u[r].y;
-        }
-        sre = wave_sum(sre);
-        sim = wave_sum(sim);
-        double energy = sqrt(sre + sim);
-        energy = energy * energy;
-        if (!(energy > thr)) continue;                             // zeroton (wave-uniform)
+        for (int r = 0; r < kRowsPerLane; ++r) u[r] = un[r];
+        todo &= todo - 1;
+        const bool more = todo != 0;
+        const int nxt = more ? __builtin_ctzll(todo) : 0;
+        if (more) load_col(un, Ub + (size_t)(base + nxt) * P, P, lane);
 
-        // --- k: one coordinate per lane
-        if (lane < nw) kk[lane] = 0;
-        wave_lds_sync();
-        for (int i0 = 1; i0 <= n; i0 += 64) {
-            const int i = i0 + lane;
-            if (i <= n) {
-                uint32_t digit;
-                if (d.method == QSFT_DETECT_NSO1) {
-                    const int p1 = d.p1, R = P / p1;
-                    double zr = 0.0, zi = 0.0;
-                    for (int r = 0; r < R; ++r) {
-                        const double2 u0 = col[r * p1];
-                        const double2 ui = col[r * p1 + i];
-                        // u0 * conj(ui), as numpy forms the complex product
-                        const double pr = __dsub_rn(__dmul_rn(u0.x, ui.x), __dmul_rn(u0.y, -ui.y));
-                        const double pi_ = __dadd_rn(__dmul_rn(u0.x, -ui.y), __dmul_rn(u0.y, ui.x));
-                        zr = __dadd_rn(zr, pr);
-                        zi = __dadd_rn(zi, pi_);
+        if (bin_energy(u) > thr) {                                 // not a zeroton (wave-uniform)
+            ++nz;
+            const int c = t / Q, j = t - c * Q;
+#pragma unroll
+            for (int r = 0; r < kRowsPerLane; ++r) {
+                const int p = lane + 64 * r;
+                if (p < P) cw[p] = u[r];
+            }
+            if (lane < nw) kk[lane] = 0;
+            wave_lds_sync();
+
+            // --- k: one coordinate per lane
+            for (int i0 = 1; i0 <= n; i0 += 64) {
+                const int i = i0 + lane;
+                if (i <= n) {
+                    uint32_t digit;
+                    if (d.method == QSFT_DETECT_NSO1) {
+                        const int p1 = d.p1, R = P / p1;
+                        double zr = 0.0, zi = 0.0;
+                        for (int r = 0; r < R; ++r) {
+                            const double2 u0 = cw[r * p1];
+                            const double2 ui = cw[r * p1 + i];
+                            // u0 * conj(ui), as numpy forms the complex product
+                            const double pr = __dsub_rn(__dmul_rn(u0.x, ui.x), __dmul_rn(u0.y, -ui.y));
+                            const double pi_ = __dadd_rn(__dmul_rn(u0.x, -ui.y), __dmul_rn(u0.y, ui.x));
+                            zr = __dadd_rn(zr, pr);
+                            zi = __dadd_rn(zi, pi_);
+                        }
+                        zr = __ddiv_rn(zr, (double)R);
+                        zi = __ddiv_rn(zi, (double)R);
+                        const double th = rem_2pi(atan2(zi, zr));
+                        const double step = twopi / (double)q;     // 2 * np.pi / q
+                        int best = 0;
+                        double bestv = fabs(__dsub_rn(0.0, th));
+                        for (int tt = 1; tt <= q; ++tt) {
+                            const double v = fabs(__dsub_rn(__dmul_rn(step, (double)tt), th));
+                            if (v < bestv) { bestv = v; best = tt; }
+                        }
+                        digit = (uint32_t)(best % q);
+                    } else {
+                        const double2 v0 = cw[0];
+                        const double2 vi = cw[i];
+                        const double a0 = atan2(v0.y, v0.x), ai = atan2(vi.y, vi.x);
+                        const double v = __ddiv_rn(__dmul_rn((double)q, __dsub_rn(ai, a0)), twopi);
+                        long long kq = (long long)rint(v) % q;
+                        if (kq < 0) kq += q;
+                        digit = (uint32_t)kq;
                     }
-                    zr = __ddiv_rn(zr, (double)R);
-                    zi = __ddiv_rn(zi, (double)R);
-                    const double th = rem_2pi(atan2(zi, zr));
-                    const double step = twopi / (double)q;         // 2 * np.pi / q
-                    int best = 0;
-                    double bestv = fabs(__dsub_rn(0.0, th));
-                    for (int tt = 1; tt <= q; ++tt) {
-                        const double v = fabs(__dsub_rn(__dmul_rn(step, (double)tt), th));
-                        if (v < bestv) { bestv = v; best = tt; }
-                    }
-                    digit = (uint32_t)(best % q);
-                } else {
-                    const double2 v0 = col[0];
-                    const double2 vi = col[i];
-                    const double a0 = atan2(v0.y, v0.x), ai = atan2(vi.y, vi.x);
-                    const double v = __ddiv_rn(__dmul_rn((double)q, __dsub_rn(ai, a0)), twopi);
-                    long long kq = (long long)rint(v) % q;
-                    if (kq < 0) kq += q;
-                    digit = (uint32_t)kq;
-                }
-                kb8[i - 1] = (int8_t)digit;
-            }
-        }
-        wave_lds_sync();
-
-        // --- signature, rho, residual (rows across lanes)
-        const int8_t* __restrict__ Dc = d.D + (size_t)c * P * n_pad;
-        double2 w[kRowsPerLane];
-        double rr = 0.0, ri = 0.0;
-#pragma unroll
-        for (int r = 0; r < kRowsPerLane; ++r) {
-            const int p = lane + 64 * r;
-            w[r] = make_double2(0.0, 0.0);
-            if (p < P) {
-                w[r] = tab[mq(dot_row(Dc + (size_t)p * n_pad, kk, nw))];
-                rr += w[r].x * u[r].x + w[r].y * u[r].y;           // conj(w) * u
-                ri += w[r].x * u[r].y - w[r].y * u[r].x;
-            }
-        }
-        rr = wave_sum(rr);
-        ri = wave_sum(ri);
-        const double rho_x = __ddiv_rn(rr, (double)P), rho_y = __ddiv_rn(ri, (double)P);
-        double qre = 0.0, qim = 0.0;
-#pragma unroll
-        for (int r = 0; r < kRowsPerLane; ++r) {
-            const int p = lane + 64 * r;
-            if (p < P) {
-                const double sx = __dsub_rn(__dmul_rn(rho_x, w[r].x), __dmul_rn(rho_y, w[r].y));
-                const double sy = __dadd_rn(__dmul_rn(rho_x, w[r].y), __dmul_rn(rho_y, w[r].x));
-                const double ex = __dsub_rn(u[r].x, sx), ey = __dsub_rn(u[r].y, sy);
-                qre += ex * ex;
-                qim += ey * ey;
-            }
-        }
-        qre = wave_sum(qre);
-        qim = wave_sum(qim);
-        double res = sqrt(qre + qim);
-        res = res * res;
-
-        // --- bin match: M_c^T k mod q == digits of j (MSB first), one digit per lane
-        bool ok = true;
-        if (lane < b) {
-            int pw = 1;
-            for (int tt = lane + 1; tt < b; ++tt) pw *= q;
-            const int dig = (j / pw) % q;
-            ok = (int)mq(dot_row(d.Mt + ((size_t)c * b + lane) * n_pad, kk, nw)) == dig;
-        }
-        const bool match = __all(ok);
-        const bool single = match && res <= thr;
-
-        if (lane == 0) {
-            enlist(d, it, t);                                     // non-zeroton: rescan next pass
-            atomicAdd(&ctr[it * QSFT_CTR_ITER_STRIDE + (single ? 0 : 1)], 1u);
-        }
-        if (!single) continue;
-        if (lane < nw) reinterpret_cast<uint32_t*>(d.kbin + (size_t)t * n_pad)[lane] = kk[lane];
-        if (lane == 0) {
-            d.rho_bin[2 * (size_t)t] = rho_x;
-            d.rho_bin[2 * (size_t)t + 1] = rho_y;
-            const uint64_t hv = hash_words(kk, nw);
-            const uint64_t tg = hv | 1ull;
-            const uint32_t mask = (1u << d.hcap_log2) - 1u;
-            uint32_t pos = (uint32_t)(hv >> 32) & mask;
-            int slot = -1;
-            for (uint32_t probe = 0; probe <= mask; ++probe) {
-                const unsigned long long old =
-                    atomicCAS((unsigned long long*)&d.tag[pos], 0ull, (unsigned long long)tg);
-                if (old == 0ull || old == tg) { slot = (int)pos; break; }
-                pos = (pos + 1) & mask;
-            }
-            if (slot < 0) {
-                atomicOr(&ctr[QSFT_CTR_ERROR], (uint32_t)QSFT_DEVERR_TABLE_FULL);
-            } else {
-                d.slot_bin[t] = slot;
-                const uint64_t tm = ((uint64_t)(it + 1) << 32) | (uint32_t)t;
-                atomicMin((unsigned long long*)&d.owner[slot], (unsigned long long)tm);
-                const unsigned long long prev = atomicMax((unsigned long long*)&d.last[slot], (unsigned long long)tm);
-                if ((prev >> 32) != (unsigned long long)(it + 1)) atomicAdd(&ctr[it * QSFT_CTR_ITER_STRIDE + 2], 1u);
-                const uint32_t rix = atomicAdd(&ctr[QSFT_CTR_LOG], 1u);
-                if ((uint64_t)rix >= (uint64_t)d.max_iter * (uint64_t)CQ) {   // cannot happen: one record per bin per pass
-                    atomicOr(&ctr[QSFT_CTR_ERROR], (uint32_t)QSFT_DEVERR_BAD_SHAPE);
-                } else {
-                    d.rec_time[rix] = tm;
-                    d.rec_rho[2 * (size_t)rix] = rho_x;
-                    d.rec_rho[2 * (size_t)rix + 1] = rho_y;
-                    d.rec_next[rix] = atomicExch(&d.head[slot], rix);
-                    atomicAdd(&d.count[slot], 1u);
-                    const uint32_t sp = atomicAdd(&ctr[it * QSFT_CTR_ITER_STRIDE + 5], 1u);
-                    d.slist[sp] = t;
+                    kb8[i - 1] = (int8_t)digit;
                 }
             }
+            wave_lds_sync();
+
+            // --- signature, rho, residual (rows across lanes)
+            const int8_t* __restrict__ Dc = d.D + (size_t)c * P * n_pad;
+            double2 w[kRowsPerLane];
+            double rr = 0.0, ri = 0.0;
+#pragma unroll
+            for (int r = 0; r < kRowsPerLane; ++r) {
+                const int p = lane + 64 * r;
+                w[r] = make_double2(0.0, 0.0);
+                if (p < P) {
+                    w[r] = tab[mq(dot_row(Dc + (size_t)p * n_pad, kk, nw))];
+                    rr += w[r].x * u[r].x + w[r].y * u[r].y;       // conj(w) * u
+                    ri += w[r].x * u[r].y - w[r].y * u[r].x;
+                }
+            }
+            rr = wave_sum(rr);
+            ri = wave_sum(ri);
+            const double rho_x = __ddiv_rn(rr, (double)P), rho_y = __ddiv_rn(ri, (double)P);
+            double qre = 0.0, qim = 0.0;
+#pragma unroll
+            for (int r = 0; r < kRowsPerLane; ++r) {
+                const int p = lane + 64 * r;
+                if (p < P) {
+                    const double sx = __dsub_rn(__dmul_rn(rho_x, w[r].x), __dmul_rn(rho_y, w[r].y));
+                    const double sy = __dadd_rn(__dmul_rn(rho_x, w[r].y), __dmul_rn(rho_y, w[r].x));
+                    const double ex = __dsub_rn(u[r].x, sx), ey = __dsub_rn(u[r].y, sy);
+                    qre += ex * ex;
+                    qim += ey * ey;
+                }
+            }
+            qre = wave_sum(qre);
+            qim = wave_sum(qim);
+            double res = sqrt(qre + qim);
+            res = res * res;
+
+            // --- bin match: M_c^T k mod q == digits of j (MSB first), one digit per lane
+            bool ok = true;
+            if (lane < b) {
+                const int dig = (j / digit_weight(q, b, lane)) % q;
+                ok = (int)mq(dot_row(d.Mt + ((size_t)c * b + lane) * n_pad, kk, nw)) == dig;
+            }
+            const bool match = __all(ok);
+            const bool single = match && res <= thr;
+
+            if (lane == 0) d.mark[t] = it + 1;                      // non-zeroton: rescanned next pass
+            if (single) {
+                if (lane < nw) reinterpret_cast<uint32_t*>(d.kbin + (size_t)t * n_pad)[lane] = kk[lane];
+                if (lane == 0) {
+                    d.rho_bin[2 * (size_t)t] = rho_x;
+                    d.rho_bin[2 * (size_t)t + 1] = rho_y;
+                    sb_t[wave][ns] = t;
+                    sb_h[wave][ns] = hash_words(kk, nw);
+                    sb_rho[wave][ns] = make_double2(rho_x, rho_y);
+                }
+                ++ns;
+            }
+            wave_lds_sync();                                       // kk / cw are rewritten by the next bin
+        }
+        if (!more) break;
+        cur = nxt;
+    }
+
+    if (lane == 0) {
+        uint32_t* pc = ctr + it * QSFT_CTR_ITER_STRIDE;
+        atomicAdd(pc + 4, scanned);
+        if (ns) atomicAdd(pc + 0, (uint32_t)ns);
+        if (nz > ns) atomicAdd(pc + 1, (uint32_t)(nz - ns));
+    }
+    if (ns == 0) return;
+
+    // --- hash registration, one lane per singleton of this wave
+    int slot = -1;
+    bool newball = false;
+    int t = 0;
+    uint64_t tm = 0;
+    double2 rho = make_double2(0.0, 0.0);
+    if (lane < ns) {
+        t = sb_t[wave][lane];
+        rho = sb_rho[wave][lane];
+        const uint64_t hv = sb_h[wave][lane];
+        const uint64_t tg = hv | 1ull;
+        const uint32_t mask = (1u << d.hcap_log2) - 1u;
+        uint32_t pos = (uint32_t)(hv >> 32) & mask;
+        for (uint32_t probe = 0; probe <= mask; ++probe) {
+            const unsigned long long old = atomicCAS((unsigned long long*)&d.tag[pos], 0ull, (unsigned long long)tg);
+            if (old == 0ull || old == tg) { slot = (int)pos; break; }
+            pos = (pos + 1) & mask;
+        }
+        if (slot < 0) {
+            atomicOr(&ctr[QSFT_CTR_ERROR], (uint32_t)QSFT_DEVERR_TABLE_FULL);
+        } else {
+            d.slot_bin[t] = slot;
+            d.sstamp[t] = it + 1;                                 // kernel B peels only registered bins
+            tm = ((uint64_t)(it + 1) << 32) | (uint32_t)t;
+            atomicMin((unsigned long long*)&d.owner[slot], (unsigned long long)tm);
+            const unsigned long long prev = atomicMax((unsigned long long*)&d.last[slot], (unsigned long long)tm);
+            newball = (prev >> 32) != (unsigned long long)(it + 1);
+        }
+    }
+    const uint64_t nb_mask = __ballot(newball);
+    const uint64_t ok_mask = __ballot(slot >= 0);
+    uint32_t rbase = 0;
+    if (lane == 0) {
+        if (nb_mask) atomicAdd(&ctr[it * QSFT_CTR_ITER_STRIDE + 2], (uint32_t)__popcll(nb_mask));
+        if (ok_mask) rbase = atomicAdd(&ctr[QSFT_CTR_LOG], (uint32_t)__popcll(ok_mask));
+    }
+    rbase = __shfl(rbase, 0);
+    if (slot >= 0) {
+        const uint32_t rix = rbase + (uint32_t)__popcll(ok_mask & ((1ull << lane) - 1ull));
+        if ((uint64_t)rix >= (uint64_t)d.max_iter * (uint64_t)CQ) {   // cannot happen: one record per bin per pass
+            atomicOr(&ctr[QSFT_CTR_ERROR], (uint32_t)QSFT_DEVERR_BAD_SHAPE);
+        } else {
+            d.rec_time[rix] = tm;
+            d.rec_rho[2 * (size_t)rix] = rho.x;
+            d.rec_rho[2 * (size_t)rix + 1] = rho.y;
+            d.rec_next[rix] = atomicExch(&d.head[slot], rix);
+            atomicAdd(&d.count[slot], 1u);
         }
     }
 }
 
+// Kernel B. Same bin partition as kernel A; the wave walks the singletons of
+// this pass among its 16 bins (sstamp == pass + 1). For each: the k must equal
+// its slot's key; the last detection of the key peels it from every group
+// (one target bin per group, C x b bin digits computed by as many lanes at
+// once) with FP64 atomics over the target's contiguous column, and stamps the
+// target for the next pass's scan.
 __global__ __launch_bounds__(kBinThreads) void k_peel_apply(const qsft_peel_desc* __restrict__ descs,
                                                             const int32_t* __restrict__ pre, int n_inst, int it) {
     __shared__ double2 tab[QSFT_MAX_Q];
@@ -336,13 +458,20 @@ __global__ __launch_bounds__(kBinThreads) void k_peel_apply(const qsft_peel_desc
     fill_wtable_d(tab, q);
     __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int CQ = C * Q;
+    const int base = (local * kWaves + wave) * kBinsPerWave;
+    if (base >= CQ) return;
+    const bool is_single = base + lane < CQ && d.sstamp[base + lane] == it + 1;
+    uint64_t todo = __ballot(is_single);
+    if (todo == 0) return;
     uint32_t* kk = kws[wave];
-    const int count = (int)ctr[it * QSFT_CTR_ITER_STRIDE + 5];
-    const int waves_inst = (pre[inst + 1] - pre[inst]) * kWaves;
     const ModQ mq((uint32_t)q);
+    const bool shared_d = (d.flags & QSFT_PEEL_SHARED_D) != 0;
+    const bool all_digits = C * b <= 64;          // every group's bin digits in one step
 
-    for (int e = local * kWaves + wave; e < count; e += waves_inst) {
-        const int t = d.slist[e];
+    while (todo) {
+        const int t = base + __builtin_ctzll(todo);
+        todo &= todo - 1;
         const int slot = d.slot_bin[t];
         const uint64_t tm = ((uint64_t)(it + 1) << 32) | (uint32_t)t;
         const uint32_t* ksrc = reinterpret_cast<const uint32_t*>(d.kbin + (size_t)t * n_pad);
@@ -362,32 +491,51 @@ __global__ __launch_bounds__(kBinThreads) void k_peel_apply(const qsft_peel_desc
             same = kslot[lane] == kk[lane];
         }
         if (!__all(same) && lane == 0) atomicOr(&ctr[QSFT_CTR_ERROR], (uint32_t)QSFT_DEVERR_TAG_COLLISION);
-        if (d.last[slot] != tm) continue;
-        // the last detection of this key in this pass peels it from every group
-        const double rx = d.rho_bin[2 * (size_t)t], ry = d.rho_bin[2 * (size_t)t + 1];
-        for (int l = 0; l < C; ++l) {
+        if (d.last[slot] == tm) {
+            // the last detection of this key in this pass peels it from every group
+            const double rx = d.rho_bin[2 * (size_t)t], ry = d.rho_bin[2 * (size_t)t + 1];
             int hd = 0;
-            if (lane < b) {
-                int pw = 1;
-                for (int tt = lane + 1; tt < b; ++tt) pw *= q;
-                hd = (int)mq(dot_row(d.Mt + ((size_t)l * b + lane) * n_pad, kk, nw)) * pw;
+            if (all_digits && lane < C * b) {
+                const int l = lane / b, dg = lane - l * b;
+                hd = (int)mq(dot_row(d.Mt + ((size_t)l * b + dg) * n_pad, kk, nw)) * digit_weight(q, b, dg);
             }
-            const int jl = wave_sum_i(hd);
-            const int8_t* __restrict__ Dl = d.D + (size_t)l * P * n_pad;
-            double* __restrict__ Ucol = d.U + 2 * ((size_t)l * Q + jl) * P;
-#pragma unroll
-            for (int r = 0; r < kRowsPerLane; ++r) {
-                const int p = lane + 64 * r;
-                if (p < P) {
-                    const double2 wv = tab[mq(dot_row(Dl + (size_t)p * n_pad, kk, nw))];
-                    const double sx = __dsub_rn(__dmul_rn(rx, wv.x), __dmul_rn(ry, wv.y));
-                    const double sy = __dadd_rn(__dmul_rn(rx, wv.y), __dmul_rn(ry, wv.x));
-                    unsafeAtomicAdd(Ucol + 2 * p, -sx);
-                    unsafeAtomicAdd(Ucol + 2 * p + 1, -sy);
+            double sx[kRowsPerLane], sy[kRowsPerLane];
+            for (int l = 0; l < C; ++l) {
+                int part;
+                if (all_digits) {
+                    part = (lane >= l * b && lane < (l + 1) * b) ? hd : 0;
+                } else {
+                    part = (lane < b) ? (int)mq(dot_row(d.Mt + ((size_t)l * b + lane) * n_pad, kk, nw)) *
+                                            digit_weight(q, b, lane)
+                                      : 0;
                 }
+                const int jl = wave_sum_i(part);
+                if (l == 0 || !shared_d) {
+                    const int8_t* __restrict__ Dl = d.D + (size_t)l * P * n_pad;
+#pragma unroll
+                    for (int r = 0; r < kRowsPerLane; ++r) {
+                        const int p = lane + 64 * r;
+                        sx[r] = sy[r] = 0.0;
+                        if (p < P) {
+                            const double2 wv = tab[mq(dot_row(Dl + (size_t)p * n_pad, kk, nw))];
+                            sx[r] = __dsub_rn(__dmul_rn(rx, wv.x), __dmul_rn(ry, wv.y));
+                            sy[r] = __dadd_rn(__dmul_rn(rx, wv.y), __dmul_rn(ry, wv.x));
+                        }
+                    }
+                }
+                double* __restrict__ Ucol = d.U + 2 * ((size_t)l * Q + jl) * P;
+#pragma unroll
+                for (int r = 0; r < kRowsPerLane; ++r) {
+                    const int p = lane + 64 * r;
+                    if (p < P) {
+                        unsafeAtomicAdd(Ucol + 2 * p, -sx[r]);
+                        unsafeAtomicAdd(Ucol + 2 * p + 1, -sy[r]);
+                    }
+                }
+                if (lane == 0) d.mark[l * Q + jl] = it + 1;        // the peeled bin is rescanned
             }
-            if (lane == 0) enlist(d, it, l * Q + jl);              // the peeled bin is rescanned
         }
+        wave_lds_sync();                                           // kk is rewritten by the next singleton
     }
 }
 
@@ -513,7 +661,7 @@ PeelLayout peel_layout(long long CQ, int n_pad, int hl2, int max_iter, int out_c
         L * 8, L * 16, L * 4,    // 9-11 rec_time rec_rho rec_next
         QSFT_CTR_WORDS * 4,      // 12 ctr
         (size_t)out_cap * n_pad, (size_t)out_cap * 16, (size_t)out_cap * 8,  // 13-15 outputs
-        (size_t)CQ * 4, (size_t)CQ * 4, (size_t)CQ * 4, (size_t)CQ * 4,      // 16-19 mark list0 list1 slist
+        (size_t)CQ * 4, (size_t)CQ * 4,                                      // 16-17 mark sstamp
     };
     PeelLayout lay{};
     size_t o = 0;
@@ -581,9 +729,8 @@ extern "C" int qsft_peel_bind(qsft_peel_desc* h, int q, int n, int b, int C, int
     d.out_val = reinterpret_cast<double*>(base + lay.off[14]);
     d.out_order = reinterpret_cast<uint64_t*>(base + lay.off[15]);
     d.mark = reinterpret_cast<int32_t*>(base + lay.off[16]);
-    d.list0 = reinterpret_cast<int32_t*>(base + lay.off[17]);
-    d.list1 = reinterpret_cast<int32_t*>(base + lay.off[18]);
-    d.slist = reinterpret_cast<int32_t*>(base + lay.off[19]);
+    d.sstamp = reinterpret_cast<int32_t*>(base + lay.off[17]);
+    d.flags = 0;
     *h = d;
     return QSFT_OK;
 }
@@ -602,11 +749,18 @@ extern "C" int qsft_peel_slot_blocks(const qsft_peel_desc* h) {
     return (int)(((H > CQ ? H : CQ) + kSlotThreads - 1) / kSlotThreads);
 }
 
-extern "C" int qsft_peel_init(const qsft_peel_desc* d, const int32_t* pre_slots, int n_inst, int blocks_slots,
-                              void* stream) {
-    if (!d || !pre_slots || n_inst < 1 || blocks_slots < 1) { qsft_set_error("qsft_peel_init: bad arguments"); return QSFT_ERR_ARG; }
-    hipLaunchKernelGGL(k_peel_init, dim3(blocks_slots), dim3(kSlotThreads), 0, (hipStream_t)stream, d, pre_slots, n_inst);
-    return qsft_hip_status(hipGetLastError(), "qsft_peel_init");
+extern "C" int qsft_peel_init(const qsft_peel_desc* d, const int32_t* pre_bins, const int32_t* pre_slots,
+                              int n_inst, int blocks_bins, int blocks_slots, void* stream) {
+    if (!d || !pre_bins || !pre_slots || n_inst < 1 || blocks_bins < 1 || blocks_slots < 1) {
+        qsft_set_error("qsft_peel_init: bad arguments");
+        return QSFT_ERR_ARG;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_peel_init, dim3(blocks_slots), dim3(kSlotThreads), 0, st, d, pre_slots, n_inst);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return qsft_hip_status(e, "qsft_peel_init");
+    hipLaunchKernelGGL(k_peel_screen, dim3(blocks_bins), dim3(kBinThreads), 0, st, d, pre_bins, n_inst);
+    return qsft_hip_status(hipGetLastError(), "qsft_peel_init(screen)");
 }
 
 extern "C" int qsft_peel_iteration(const qsft_peel_desc* d, const int32_t* pre_bins, int n_inst, int blocks_bins,
@@ -633,7 +787,7 @@ extern "C" int qsft_peel_finalize(const qsft_peel_desc* d, const int32_t* pre_sl
 extern "C" int qsft_peel_run(const qsft_peel_desc* d, const int32_t* pre_bins, const int32_t* pre_slots, int n_inst,
                              int blocks_bins, int blocks_slots, int max_iter, void* stream) {
     if (max_iter < 1 || max_iter > 15) { qsft_set_error("qsft_peel_run: max_iter must be in [1, 15]"); return QSFT_ERR_ARG; }
-    int rc = qsft_peel_init(d, pre_slots, n_inst, blocks_slots, stream);
+    int rc = qsft_peel_init(d, pre_bins, pre_slots, n_inst, blocks_bins, blocks_slots, stream);
     for (int it = 0; rc == QSFT_OK && it < max_iter; ++it) rc = qsft_peel_iteration(d, pre_bins, n_inst, blocks_bins, it, stream);
     if (rc == QSFT_OK) rc = qsft_peel_finalize(d, pre_slots, n_inst, blocks_slots, stream);
     return rc;

@@ -73,6 +73,23 @@ __device__ __forceinline__ double2 normal2(uint64_t counter, uint64_t seed) {
     return make_double2(rad * c, rad * s);
 }
 
+// Four independent N(0, 1) draws from one Philox block: Box-Muller in FP32 on
+// two pairs of 24-bit uniforms, u1 in (0, 1) so the log is finite (largest
+// radius sqrt(50 ln 2) = 5.9). v_log_f32 is log2 and v_sin_f32 / v_cos_f32
+// take revolutions, so the angle needs no 2 pi scaling.
+__device__ __forceinline__ float4 normal4(uint64_t counter, uint64_t seed) {
+    const u4 r = philox4x32_10(u4{(uint32_t)counter, (uint32_t)(counter >> 32), 0u, 0u},
+                               (uint32_t)seed, (uint32_t)(seed >> 32));
+    const float s24 = 0x1.0p-24f;
+    const float u1a = ((float)(r.x >> 8) + 0.5f) * s24, u2a = (float)(r.y >> 8) * s24;
+    const float u1b = ((float)(r.z >> 8) + 0.5f) * s24, u2b = (float)(r.w >> 8) * s24;
+    const float m2ln2 = -1.3862943611198906f;                      // -2 ln 2
+    const float ra = __builtin_sqrtf(m2ln2 * __builtin_amdgcn_logf(u1a));
+    const float rb = __builtin_sqrtf(m2ln2 * __builtin_amdgcn_logf(u1b));
+    return make_float4(ra * __builtin_amdgcn_cosf(u2a), ra * __builtin_amdgcn_sinf(u2a),
+                       rb * __builtin_amdgcn_cosf(u2b), rb * __builtin_amdgcn_sinf(u2b));
+}
+
 // 64-bit mix of a digit vector (FNV-1a over the bytes, then the splitmix64
 // finaliser); equal vectors give equal hashes.
 __device__ __forceinline__ uint64_t mix64(uint64_t z) {

@@ -134,7 +134,7 @@ class Batch:
             asm.append(item)
             pl.append(dict(q=sp.q, n=sp.n, b=sp.b, C=C, P=P, p1=sp.n + 1,
                            method=L.QSFT_DETECT_NSO1 if sp.method == "nso" else L.QSFT_DETECT_NOISELESS,
-                           cutoff=sp.threshold(), Mt=Mt, D=Dd, U=U))
+                           cutoff=sp.threshold(), Mt=Mt, D=Dd, U=U, shared_d=True))
         self.eval = K.EvalMfma(ev) if eval_mode == "mfma" else K.EvalCosets(ev, code)
         self.dft = K.Dft(df, code)
         self.asm = K.Assemble(asm)

@@ -329,7 +329,8 @@ class PeelBatch:
     """Bound peel descriptors + workspaces for a list of instances.
 
     items: dicts q, n, b, C, P, p1, method, cutoff, Mt, D, U (complex128,
-    bin-major [C][Q][P]).
+    bin-major [C][Q][P]), optional shared_d (all groups use the delay rows of
+    group 0; compared on the device when absent).
     """
 
     def __init__(self, items, max_iter=15):
@@ -349,6 +350,11 @@ class PeelBatch:
             L.check(lib.qsft_peel_bind(ctypes.byref(d), q, n, b, C, P, it["p1"], it["method"], max_iter,
                                        float(it["cutoff"]), 0, _ptr(it["Mt"]), _ptr(it["D"]), _ptr(it["U"]),
                                        _ptr(ws), nbytes, n_pad), "qsft_peel_bind")
+            shared = it.get("shared_d")
+            if shared is None:
+                Dg = it["D"]
+                shared = all(torch.equal(Dg[0], Dg[c]) for c in range(1, C))
+            d.flags = L.QSFT_PEEL_SHARED_D if shared else 0
             self.descs.append(d)
             self.ws.append(ws)
         self.bins = [lib.qsft_peel_bin_blocks(ctypes.byref(d)) for d in self.descs]
@@ -370,7 +376,7 @@ class PeelBatch:
             L.check(lib.qsft_peel_run(d, pb, ps, n, self.tot_b, self.tot_s, self.max_iter, st), "qsft_peel_run")
         else:
             # host-driven loop: one sync per pass to read the stop flag (the plain path)
-            L.check(lib.qsft_peel_init(d, ps, n, self.tot_s, st), "qsft_peel_init")
+            L.check(lib.qsft_peel_init(d, pb, ps, n, self.tot_b, self.tot_s, st), "qsft_peel_init")
             for it in range(self.max_iter):
                 L.check(lib.qsft_peel_iteration(d, pb, n, self.tot_b, it, st), "qsft_peel_iteration")
                 if it + 1 >= self.max_iter:

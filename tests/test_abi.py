@@ -29,7 +29,7 @@ def test_every_declared_symbol_is_exported_and_bound():
 def test_struct_layouts_match():
     for which, cls in L.STRUCTS.items():
         assert L.lib.qsft_sizeof(which) == ctypes.sizeof(cls)
-    assert L.lib.qsft_abi_version() == 1
+    assert L.lib.qsft_abi_version() == 2
 
 
 @pytest.mark.parametrize("q,b,prec,passes", [(4, 6, L.QSFT_F32, 1), (4, 6, L.QSFT_F64, 2), (4, 7, L.QSFT_F32, 2),

@@ -166,9 +166,13 @@ def test_dft_strided_sub_b(q, b, bb):
     assert rel_err(got, ref) < 1e-13
 
 
-def test_assemble_gather_and_noise_array():
+@pytest.mark.parametrize("p1,Q", [(5, 64), (43, 27), (65, 20)])
+def test_assemble_gather_and_noise_array(p1, Q):
+    """Exact gather + FP64 noise add; (43, 27): 86 of 129 source rows, odd Q;
+    (65, 20): 130 rows, so the bin-major tiles have a 2-row second chunk."""
+    print(f"assemble array p1={p1} Q={Q}", flush=True)
     rng = np.random.default_rng(12)
-    Csrc, Rsrc, p1, Q = 3, 2, 5, 64
+    Csrc, Rsrc = 3, 2
     src = rng.normal(size=(Csrc, Rsrc * p1, Q)) + 1j * rng.normal(size=(Csrc, Rsrc * p1, Q))
     noise = rng.normal(size=(2, 2 * p1, Q)) + 1j * rng.normal(size=(2, 2 * p1, Q))
     dev = dv.current_device()
@@ -203,3 +207,22 @@ def test_assemble_philox_noise_statistics():
     K.assemble([dict(Q=Q, C=C, R=R, p1=p1, src=src, src_rows=R * p1, gsel=[0, 1, 2], rsel=[0, 1], U=U2,
                      sigma=sigma, seed=1234)])
     assert torch.equal(U, U2)                      # counter-based: same seed, same draws
+
+
+@pytest.mark.parametrize("R,p1,Q", [(2, 51, 4096), (3, 43, 27), (1, 129, 20)])
+def test_assemble_philox_layout_independent(R, p1, Q):
+    """The bin-major store draws exactly the row-major noise (the noise is a
+    function of the logical element), including odd Q and a 1-row chunk."""
+    print(f"assemble philox layouts R={R} p1={p1} Q={Q}", flush=True)
+    dev = dv.current_device()
+    C = 2
+    rng = np.random.default_rng(5)
+    src = torch.from_numpy(rng.normal(size=(C, R * p1, Q)) + 1j * rng.normal(size=(C, R * p1, Q)))
+    src = src.to(torch.complex64).to(dev)
+    args = dict(Q=Q, C=C, R=R, p1=p1, src=src, src_rows=R * p1, gsel=[1, 0], rsel=list(range(R))[::-1],
+                sigma=0.8, seed=99)
+    Ur = torch.empty((C, R * p1, Q), dtype=torch.complex128, device=dev)
+    Ub = torch.empty((C, Q, R * p1), dtype=torch.complex128, device=dev)
+    K.assemble([dict(args, U=Ur, bin_major=False)])
+    K.assemble([dict(args, U=Ub, bin_major=True)])
+    assert torch.equal(Ub.transpose(1, 2), Ur)
