@@ -85,7 +85,8 @@ int qsft_eval_cosets(const qsft_eval_desc* dev_descs, const int32_t* dev_pre, in
  * <(M_c l + d_p) mod q, k_s> = <(M_c l) mod q, k_s> + <d_p, k_s> (mod q), so
  * x[c][p][l] = sum_s W_c[l][s] B_c[s][p] with W_c = w^{E0} (E0 from an int8
  * MFMA over the q^b base queries) and B_c[s][p] = a_s w^{<d_p, k_s>}: a complex
- * GEMM on bf16 MFMA with B split into three bf16 limbs and FP32 accumulation.
+ * GEMM on fp16 MFMA with 2^bexp B split into two fp16 limbs and FP32
+ * accumulation (x is scaled back by 2^-bexp).
  * A prep kernel writes B_c into `Bf` (qsft_eval_mfma_workspace_bytes(C, P, S)
  * bytes) before the main kernel. Replaces the same reference code as
  * qsft_eval_cosets. */
@@ -95,6 +96,9 @@ typedef struct qsft_eval_mfma_desc {
     int32_t nsb;            /* (S + 15) / 16  support blocks                                */
     int32_t npt;            /* (P + 7) / 8    delay-row tiles                               */
     int32_t shared_d;       /* 1: every group has the same delay rows (B computed once)     */
+    int32_t bexp;           /* 14 - e where max|a_s| = m 2^e, m in [0.5, 1): the largest    *
+                             * scaled |B| lies in [2^13, 2^14); in [-100, 100]               */
+    int32_t _pad1;
     const int8_t* Mt;       /* [C][b][n_pad]                                                */
     const int8_t* D;        /* [C][P][n_pad]                                                */
     const int8_t* K;        /* [S][n_pad]                                                   */

@@ -9,26 +9,29 @@
 //   B_c[s][p] = a_s w^{<d_p, k_s> mod q}.
 // Each sample is still the full S-term sum of a_s w^{<m, k_s>}; the B*S
 // complex multiply-adds run as a (Q x S) x (S x P) complex GEMM on
-// v_mfma_f32_16x16x32_bf16, and the w-table epilogue runs C*Q*S times
+// v_mfma_f32_16x16x32_f16, and the w-table epilogue runs C*Q*S times
 // instead of C*P*Q*S times.
 //
 //   * E0 comes from v_mfma_i32_16x16x64_i8 (support digits x base-query digits,
 //     exact int32); its D fragment (lane = query column l, 4 rows = supports
-//     4g..4g+3) is exactly the bf16 A fragment needed next: slots 0..3 hold
+//     4g..4g+3) is exactly the f16 A fragment needed next: slots 0..3 hold
 //     Re W for those 4 supports and slots 4..7 Im W.
-//   * B is precomputed (prep kernel) in FP64, split into three bf16 limbs
-//     (hi + mid + lo carries ~24 significant bits), and stored already in the
-//     B-fragment order of the bf16 MFMA, with the complex product laid out as
+//   * B is precomputed (prep kernel) in FP64, scaled by 2^bexp so that its
+//     largest entry lies in [2^13, 2^14), split into two fp16 limbs (hi + lo:
+//     the lo limb carries the rounding error of hi, ~22 significant bits
+//     relative to max|B| and never overflowing), and stored already in the
+//     B-fragment order of the f16 MFMA, with the complex product laid out as
 //     a real GEMM: column 2p' holds Re x, column 2p'+1 holds Im x, and the
 //     "Im W" k-slots carry -Im B / Re B. When all groups share their delay
 //     rows (the reference's get_Ms_and_Ds), B is computed once.
-//   * w^e is exact in bf16 for q in {2, 4} (0, +-1); other q split W into
-//     three limbs too and keep the six limb products of order <= 2.
-//   * accumulation is FP32 in the MFMA; x is written as complex FP32.
+//   * w^e is exact in fp16 for q in {2, 4} (0, +-1); other q split W into
+//     two limbs too and keep the three limb products of order <= 1.
+//   * accumulation is FP32 in the MFMA; x is scaled back by 2^-bexp (exact)
+//     and written as complex FP32.
 //
 // A workgroup owns 256 base queries (4 waves x 4 subtiles of 16) x 8 p-tiles
-// (64 delay rows) of one group. Per 16-support block it stages the 8 x 3 B
-// fragments (24 KiB) in LDS and each wave reuses every fragment for its 4
+// (64 delay rows) of one group. Per 16-support block it stages the 8 x 2 B
+// fragments (16 KiB) in LDS and each wave reuses every fragment for its 4
 // subtiles; the next block's fragments are loaded into registers while the
 // current block's MFMAs run. Logical blocks are dealt out XCD by XCD so the
 // workgroups that share B fragments share an L2.
@@ -39,34 +42,28 @@ namespace qsft {
 typedef int v4i __attribute__((ext_vector_type(4)));
 typedef unsigned int v4u __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
-typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+typedef _Float16 v8h __attribute__((ext_vector_type(8)));
 
 constexpr int kMfThreads = 256;
 constexpr int kRowsPerBlock = 256;   // base queries per workgroup
 constexpr int kPtPerBlock = 8;       // p-tiles (8 complex delay rows each) per workgroup
-constexpr int kLimbs = 3;            // bf16 limbs of B
+constexpr int kLimbs = 2;            // fp16 limbs of B (and of W for q not in {2, 4})
 constexpr int kStage = kPtPerBlock * kLimbs * 64;      // v4u per staged support block
-constexpr int kStagePerThread = kStage / kMfThreads;   // 6
+constexpr int kStagePerThread = kStage / kMfThreads;   // 4
 constexpr int kXcds = 8;
 
-// round-to-nearest-even float -> bf16 bits
-__host__ __device__ __forceinline__ uint32_t bf16_bits(float f) {
-    uint32_t u = __builtin_bit_cast(uint32_t, f);
-    if ((u & 0x7F800000u) == 0x7F800000u) return u >> 16;        // inf / nan (not produced here)
-    u += 0x7FFFu + ((u >> 16) & 1u);
-    return u >> 16;
+// fp16 bits of f (round to nearest even)
+__device__ __forceinline__ uint32_t f16_bits(float f) {
+    return (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)f);
 }
-__host__ __device__ __forceinline__ double bf16_to_double(uint32_t b) {
-    return (double)__builtin_bit_cast(float, b << 16);
+__device__ __forceinline__ double f16_to_double(uint32_t b) {
+    return (double)__builtin_bit_cast(_Float16, (uint16_t)b);
 }
-// v = limb0 + limb1 + limb2 (+ O(2^-27 |v|))
-__host__ __device__ __forceinline__ void split3(double v, uint32_t (&l)[3]) {
-    double r = v;
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-        l[i] = bf16_bits((float)r);
-        r -= bf16_to_double(l[i]);
-    }
+// v = hi + lo + err, |err| <= 2^-11 |v - hi| <= 2^-22 |v| (for |v| < 2^15 and
+// |v - hi| above the fp16 subnormal range; below it err is < 2^-25 absolute)
+__device__ __forceinline__ void split2(double v, uint32_t (&l)[2]) {
+    l[0] = f16_bits((float)v);
+    l[1] = f16_bits((float)(v - f16_to_double(l[0])));
 }
 
 __device__ __forceinline__ v4i pack16b(const int (&d)[16]) {
@@ -89,8 +86,8 @@ __device__ __forceinline__ int xcd_logical_block() {
     return (xcd < rem) ? xcd * (per + 1) + slot : rem * (per + 1) + (xcd - rem) * per + slot;
 }
 
-// Prep: B_c[s][p] = a_s w^{<d_cp, k_s>} in bf16-limb B-fragment order.
-// Fragment (c, sb, pt, limb) is 64 lanes x 8 bf16; lane = 16 g + n with
+// Prep: B_c[s][p] = 2^bexp a_s w^{<d_cp, k_s>} in fp16-limb B-fragment order.
+// Fragment (c, sb, pt, limb) is 64 lanes x 8 fp16; lane = 16 g + n with
 // n = 2 (p % 8) + {0: Re column, 1: Im column}; slot j < 4 <-> "Re W" row of
 // support s = 16 sb + 4 g + j, slot 4 + j <-> "Im W" row of the same support.
 // One thread fills lanes (16 g + 2 pl, 16 g + 2 pl + 1) of every limb for one
@@ -121,7 +118,8 @@ __global__ __launch_bounds__(256) void k_eval_mfma_prep(const qsft_eval_mfma_des
     const int sb = (int)(rem / npt);
     const int p = 8 * pt + pl;
     const ModQ mq((uint32_t)q);
-    uint32_t lr[4][3], li[4][3];
+    const int bexp = d.bexp;
+    uint32_t lr[4][kLimbs], li[4][kLimbs];
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
         const int s = 16 * sb + 4 * g + j;
@@ -133,11 +131,11 @@ __global__ __launch_bounds__(256) void k_eval_mfma_prep(const qsft_eval_mfma_des
             for (int w = 0; w < (n_pad >> 2); ++w) acc = __builtin_amdgcn_sdot4(dr[w], kr[w], acc, false);
             const double2 wv = tab[mq((uint32_t)acc)];
             const double ar = d.a[2 * s], ai = d.a[2 * s + 1];
-            br = ar * wv.x - ai * wv.y;
-            bi = ar * wv.y + ai * wv.x;
+            br = ldexp(ar * wv.x - ai * wv.y, bexp);
+            bi = ldexp(ar * wv.y + ai * wv.x, bexp);
         }
-        split3(br, lr[j]);
-        split3(bi, li[j]);
+        split2(br, lr[j]);
+        split2(bi, li[j]);
     }
     v4u* __restrict__ out = reinterpret_cast<v4u*>(d.Bf);
 #pragma unroll
@@ -155,14 +153,14 @@ __global__ __launch_bounds__(256) void k_eval_mfma_prep(const qsft_eval_mfma_des
     }
 }
 
-// MODE 0: q in {2, 4} (w^e exact in bf16, q a power of two)
-// MODE 1: other powers of two (3-limb W)   MODE 2: any other q (3-limb W)
+// MODE 0: q in {2, 4} (w^e exact in fp16, q a power of two)
+// MODE 1: other powers of two (2-limb W)   MODE 2: any other q (2-limb W)
 template <int MODE, int NKB>
 __device__ __forceinline__ void eval_mfma_body(const qsft_eval_mfma_desc* __restrict__ descs,
                                                const int32_t* __restrict__ pre, int n_inst) {
-    constexpr int WL = (MODE == 0) ? 1 : 3;
-    __shared__ __align__(16) v4u bsh[kStage];                     // 24 KiB of B fragments
-    __shared__ uint32_t wtab[3][QSFT_MAX_Q];                      // (Re | Im << 16) bf16 bits per limb
+    constexpr int WL = (MODE == 0) ? 1 : kLimbs;
+    __shared__ __align__(16) v4u bsh[kStage];                     // 16 KiB of B fragments
+    __shared__ uint32_t wtab[kLimbs][QSFT_MAX_Q];                 // (Re | Im << 16) fp16 bits per limb
 
     const int lblock = xcd_logical_block();
     const int inst = find_inst(pre, n_inst, lblock);
@@ -184,10 +182,10 @@ __device__ __forceinline__ void eval_mfma_body(const qsft_eval_mfma_desc* __rest
     for (int r = threadIdx.x; r < q; r += blockDim.x) {
         double s, cc;
         sincospi(2.0 * (double)r / (double)q, &s, &cc);
-        uint32_t lc[3], ls[3];
-        split3(cc, lc);
-        split3(s, ls);
-        for (int i = 0; i < 3; ++i) wtab[i][r] = lc[i] | (ls[i] << 16);
+        uint32_t lc[kLimbs], ls[kLimbs];
+        split2(cc, lc);
+        split2(s, ls);
+        for (int i = 0; i < kLimbs; ++i) wtab[i][r] = lc[i] | (ls[i] << 16);
     }
 
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
@@ -258,7 +256,8 @@ __device__ __forceinline__ void eval_mfma_body(const qsft_eval_mfma_desc* __rest
     }
     __syncthreads();     // fragments of block 0 and wtab visible
 
-    for (int sb = 0; sb < nsb; ++sb) {
+    int sb = 0;
+    do {                 // nsb >= 1 (S >= 1 is checked on the host)
         // W fragments for the 4 subtiles of this support block
         v4u wa[4][WL];
 #pragma unroll
@@ -275,8 +274,8 @@ __device__ __forceinline__ void eval_mfma_body(const qsft_eval_mfma_desc* __rest
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
                     const uint32_t ee = (q == 2) ? 2 * r[v] : r[v];
-                    wr[v] = (ee == 0) ? 0x3F80u : (ee == 2 ? 0xBF80u : 0u);
-                    wi[v] = (ee == 1) ? 0x3F80u : (ee == 3 ? 0xBF80u : 0u);
+                    wr[v] = (ee == 0) ? 0x3C00u : (ee == 2 ? 0xBC00u : 0u);   // fp16 +-1
+                    wi[v] = (ee == 1) ? 0x3C00u : (ee == 3 ? 0xBC00u : 0u);
                 }
                 wa[t][0] = v4u{wr[0] | (wr[1] << 16), wr[2] | (wr[3] << 16), wi[0] | (wi[1] << 16), wi[2] | (wi[3] << 16)};
             } else {
@@ -306,14 +305,14 @@ __device__ __forceinline__ void eval_mfma_body(const qsft_eval_mfma_desc* __rest
         for (int wl = 0; wl < WL; ++wl) {
 #pragma unroll
             for (int bl = 0; bl < kLimbs; ++bl) {
-                if (wl + bl > 2) continue;                  // drop limb products below 2^-24
+                if (wl + bl > 1) continue;                  // drop the lo x lo product (~2^-22 relative)
 #pragma unroll
                 for (int pt = 0; pt < kPtPerBlock; ++pt) {
-                    const v8bf bf = __builtin_bit_cast(v8bf, bsh[(pt * kLimbs + bl) * 64 + lane]);
+                    const v8h bf = __builtin_bit_cast(v8h, bsh[(pt * kLimbs + bl) * 64 + lane]);
 #pragma unroll
                     for (int t = 0; t < 4; ++t)
-                        acc[t][pt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(v8bf, wa[t][wl]), bf,
-                                                                           acc[t][pt], 0, 0, 0);
+                        acc[t][pt] = __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(v8h, wa[t][wl]), bf,
+                                                                          acc[t][pt], 0, 0, 0);
                 }
             }
         }
@@ -323,12 +322,13 @@ __device__ __forceinline__ void eval_mfma_body(const qsft_eval_mfma_desc* __rest
             for (int i = 0; i < kStagePerThread; ++i) bsh[tid + i * kMfThreads] = stage[i];
         }
         __syncthreads();     // next block's fragments visible
-    }
+    } while (++sb < nsb);
 
     // store: lane holds column n = r16 -> delay row p = 8 (pt0 + pt) + n / 2, Re (n even) / Im (n odd),
     // for base queries l = lbase + 16 t + 4 g + v. Pair Re/Im across lanes (xor 1).
     float2* __restrict__ x = reinterpret_cast<float2*>(d.x) + (size_t)c * P * Q;
     const bool odd = r16 & 1;
+    const float unscale = ldexpf(1.0f, -d.bexp);                  // exact: 2^-bexp is a normal float
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
         const int l0 = lbase + 16 * t + 4 * g;
@@ -343,11 +343,11 @@ __device__ __forceinline__ void eval_mfma_body(const qsft_eval_mfma_desc* __rest
                 if (p < P) {
                     float2* row = x + (size_t)p * Q;
                     if (!odd) {
-                        if (l0 < Q) row[l0] = make_float2(a[0], o[0]);
-                        if (l0 + 1 < Q) row[l0 + 1] = make_float2(a[1], o[1]);
+                        if (l0 < Q) row[l0] = make_float2(a[0] * unscale, o[0] * unscale);
+                        if (l0 + 1 < Q) row[l0 + 1] = make_float2(a[1] * unscale, o[1] * unscale);
                     } else {
-                        if (l0 + 2 < Q) row[l0 + 2] = make_float2(o[2], a[2]);
-                        if (l0 + 3 < Q) row[l0 + 3] = make_float2(o[3], a[3]);
+                        if (l0 + 2 < Q) row[l0 + 2] = make_float2(o[2] * unscale, a[2] * unscale);
+                        if (l0 + 3 < Q) row[l0 + 3] = make_float2(o[3] * unscale, a[3] * unscale);
                     }
                 }
             }
@@ -381,8 +381,9 @@ extern "C" int qsft_hip_status(hipError_t e, const char* where);
 
 static bool mfma_desc_ok(const qsft_eval_mfma_desc* h) {
     return h && h->q >= 2 && h->q <= QSFT_MAX_Q && h->b >= 1 && h->b <= QSFT_MAX_B && h->n >= 1 && h->n <= h->n_pad &&
-           h->n_pad % 64 == 0 && h->n_pad <= QSFT_MAX_N && h->C >= 1 && h->P >= 1 && h->S >= 0 &&
-           h->nsb == (h->S + 15) / 16 && h->npt == (h->P + 7) / 8 && (h->shared_d == 0 || h->shared_d == 1);
+           h->n_pad % 64 == 0 && h->n_pad <= QSFT_MAX_N && h->C >= 1 && h->P >= 1 && h->S >= 1 &&
+           h->nsb == (h->S + 15) / 16 && h->npt == (h->P + 7) / 8 && (h->shared_d == 0 || h->shared_d == 1) &&
+           h->bexp >= -100 && h->bexp <= 100;
 }
 
 extern "C" size_t qsft_eval_mfma_workspace_bytes(int C, int P, int S) {

@@ -125,7 +125,7 @@ class Batch:
             U = torch.empty((C, Q, P), dtype=torch.complex128, device=dev)      # bin-major for the peeler
             self.bufs.append(dict(Mt=Mt, D=Dd, K=Kd, a=a, x=x, U=U))
             ev.append(dict(q=sp.q, n=sp.n, b=sp.b, C=C, P=P, S=sp.S, Mt=Mt, D=Dd, K=Kd, a=a, a64=a, x=x,
-                           shared_d=True))
+                           shared_d=True, amax=float(np.max(np.abs(sp.strengths))) if sp.S else 0.0))
             df.append(dict(q=sp.q, b=sp.b, rows=C * P, inp=x, out=x))
             item = dict(Q=Q, C=C, R=P // (sp.n + 1), p1=sp.n + 1, src=x, src_rows=P,
                         gsel=list(range(C)), rsel=list(range(P // (sp.n + 1))), U=U, bin_major=True)

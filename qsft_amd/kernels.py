@@ -11,6 +11,7 @@ block count for the evaluator, q for the DFT radix). The lower-case
 functions are one-shot conveniences (prepare + launch).
 """
 import ctypes
+import math
 from collections import defaultdict
 
 import numpy as np
@@ -109,23 +110,35 @@ def eval_cosets(items, precision):
 
 
 def mfma_mode(q):
-    """0: q in {2, 4} (w^e exact in bf16); 1: other powers of two; 2: any other q."""
+    """0: q in {2, 4} (w^e exact in fp16); 1: other powers of two; 2: any other q."""
     if q in (2, 4):
         return 0
     return 1 if is_pow2(q) else 2
 
 
+def mfma_bexp(amax, a64):
+    """Scale exponent of the fp16 limbs: 14 - e with max|a| = m 2^e, m in [0.5, 1)."""
+    if amax is None:
+        amax = float(a64.abs().max().item()) if a64.numel() else 0.0
+    if not amax > 0.0:
+        return 0
+    return int(min(100, max(-100, 14 - math.frexp(amax)[1])))
+
+
 class EvalMfma:
     """Matrix-core coset evaluation (FP32 out).
 
-    items: dicts q, n, b, C, P, S, Mt, D, K, a64 (complex128 [S]), x (complex64 [C][P][q^b]).
+    items: dicts q, n, b, C, P, S, Mt, D, K, a64 (complex128 [S]), x (complex64 [C][P][q^b]),
+    optional amax (max |a_s|, else read from a64) and shared_d.
     The B-fragment workspace is allocated here, one per instance.
     """
 
     def __init__(self, items):
         groups = defaultdict(list)
+        self.empty = [it["x"] for it in items if it["S"] == 0]     # no support: x = 0
         for it in items:
-            groups[(it["Mt"].shape[-1] // 64, mfma_mode(it["q"]))].append(it)
+            if it["S"] > 0:
+                groups[(it["Mt"].shape[-1] // 64, mfma_mode(it["q"]))].append(it)
         self.launches = []
         self.ws = []
         for (nkb, mode), its in groups.items():
@@ -143,6 +156,7 @@ class EvalMfma:
                 self.ws.append(ws)
                 d = L.EvalMfmaDesc(q=it["q"], n=it["n"], b=it["b"], C=C, P=P, S=S, n_pad=it["Mt"].shape[-1],
                                    nsb=(S + 15) // 16, npt=(P + 7) // 8, shared_d=int(bool(shared)),
+                                   bexp=mfma_bexp(it.get("amax", None), it["a64"]),
                                    Mt=_ptr(it["Mt"]), D=_ptr(it["D"]), K=_ptr(it["K"]), a=_ptr(it["a64"]),
                                    Bf=_ptr(ws), x=_ptr(it["x"]))
                 pb, mb = ctypes.c_int(), ctypes.c_int()
@@ -155,6 +169,8 @@ class EvalMfma:
             self.launches.append((buf, dptr, pp, pm, len(descs), tp, tm, (nkb << 4) | (mode << 8)))
 
     def launch(self):
+        for x in self.empty:
+            x.zero_()
         for _, dptr, pp, pm, n, tp, tm, word in self.launches:
             L.check(lib.qsft_eval_mfma(_vp(dptr), _vp(pp), _vp(pm), n, tp, tm, word, stream_handle()),
                     "qsft_eval_mfma")
@@ -162,6 +178,8 @@ class EvalMfma:
     def launch_split(self, between):
         """The B-fragment prep launches, then `between()`, then the GEMM
         launches (so events can time the GEMM kernel on its own)."""
+        for x in self.empty:
+            x.zero_()
         st = stream_handle()
         for _, dptr, pp, pm, n, tp, tm, word in self.launches:
             L.check(lib.qsft_eval_mfma(_vp(dptr), _vp(pp), _vp(pm), n, tp, 0, word, st), "qsft_eval_mfma(prep)")

@@ -68,7 +68,7 @@ MFMA_CASES = CASES + [(8, 30, 3, 77, 2, 2), (2, 250, 4, 33, 3, 1), (4, 64, 5, 12
 
 @pytest.mark.parametrize("q,n,b,S,C,R", MFMA_CASES)
 def test_eval_mfma_matches_oracle(q, n, b, S, C, R):
-    """Matrix-core evaluation (bf16x3 limbs, FP32 accumulation) against the
+    """Matrix-core evaluation (fp16x2 limbs, FP32 accumulation) against the
     oracle's per-query FP64 evaluation, every row."""
     print(f"eval_mfma q={q} n={n} b={b} S={S} C={C} R={R}", flush=True)
     Ms, D, locq, a = _setup(q, n, b, S, C, R, seed=q * 1000 + n + 7)
@@ -86,6 +86,29 @@ def test_eval_mfma_matches_oracle(q, n, b, S, C, R):
             dig = O.coset_digits(Ms[c], D[p:p + 1], q, b)[0]
             ref = O.eval_synthetic(dig, locq, a, q)
             assert rel_err(x[c, p], ref) < 2e-6, (c, p)
+
+
+@pytest.mark.parametrize("q,scale", [(4, 1e-7), (4, 3e6), (3, 1e-5), (4, 0.0)])
+def test_eval_mfma_strength_scale(q, scale):
+    """The fp16 limbs are scaled by 2^bexp from max|a|: tiny and huge
+    strengths keep the FP32-level error; scale 0 also covers S = 0."""
+    print(f"eval_mfma scale q={q} scale={scale}", flush=True)
+    n, b, C, R = 12, 3, 2, 2
+    S = 0 if scale == 0.0 else 70
+    Ms, D, locq, a = _setup(q, n, b, max(S, 1), C, R, seed=77)
+    locq, a = locq[:, :S], a[:S] * scale
+    dev = dv.current_device()
+    n_pad = dv.n_pad_of(n)
+    x = dv.evaluate_cosets_mfma(q, n, b, dv.Mt_tensor(Ms, b, n_pad, dev), dv.D_tensor([D] * C, n_pad, dev),
+                                dv.digits_tensor(locq.T, n_pad, dev), dv.strengths_tensor(a, "fp64", dev))
+    x = x.cpu().numpy()
+    if S == 0:
+        assert not np.any(x)
+        return
+    for c in range(C):
+        dig = O.coset_digits(Ms[c], D, q, b)
+        ref = O.eval_synthetic(dig.reshape(-1, n), locq, a, q).reshape(D.shape[0], q ** b)
+        assert rel_err(x[c], ref) < 2e-6, c
 
 
 @pytest.mark.parametrize("q", [4, 3])
