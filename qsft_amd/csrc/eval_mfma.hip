@@ -32,9 +32,9 @@
 // A workgroup owns 256 base queries (4 waves x 4 subtiles of 16) x 8 p-tiles
 // (64 delay rows) of one group. Per 16-support block it stages the 8 x 2 B
 // fragments (16 KiB) in LDS and each wave reuses every fragment for its 4
-// subtiles; the next block's fragments are loaded into registers while the
-// current block's MFMAs run. Logical blocks are dealt out XCD by XCD so the
-// workgroups that share B fragments share an L2.
+// subtiles; the fragments of the next three blocks are loaded straight into
+// LDS (a four-slot ring) while the current block's MFMAs run. Logical blocks are
+// dealt out XCD by XCD so the workgroups that share B fragments share an L2.
 #include "qsft_common.h"
 
 namespace qsft {
@@ -153,13 +153,70 @@ __global__ __launch_bounds__(256) void k_eval_mfma_prep(const qsft_eval_mfma_des
     }
 }
 
+// s_waitcnt immediate: vector-memory count <= n, the other counters unconstrained (gfx9 encoding)
+template <int N>
+__device__ __forceinline__ void wait_vm() {
+    static_assert(N >= 0 && N < 64, "vmcnt range");
+    __builtin_amdgcn_s_waitcnt((N & 0xF) | ((N >> 4) << 14) | (0x7 << 4) | (0xF << 8));
+}
+
+// LDS byte address of a __shared__ object
+__device__ __forceinline__ uint32_t lds_addr(const void* p) {
+    return (uint32_t)(uintptr_t)(const __attribute__((address_space(3))) char*)p;
+}
+
+// 16 bytes per lane from global memory straight into LDS: lane L writes
+// lds_base + 16 L (lds_base wave-uniform). Issued from asm so the compiler
+// does not track the LDS write: its own tracking cannot tell the ring slots
+// apart and would wait for these loads before every read of another slot.
+// The instruction writes no VGPR; m0 is read when it issues and is restored
+// right after. Every read of a destination slot follows an explicit wait_vm.
+// The global address is a wave-uniform base (SGPR pair) plus a 32-bit
+// per-lane byte offset.
+__device__ __forceinline__ void dma16(const void* sbase, uint32_t voff, uint32_t lds_base) {
+    uint32_t saved;
+    asm volatile(
+        "s_mov_b32 %0, m0\n\t"
+        "s_mov_b32 m0, %3\n\t"
+        "s_nop 0\n\t"
+        "global_load_lds_dwordx4 %1, %2\n\t"
+        "s_mov_b32 m0, %0"
+        : "=&s"(saved)
+        : "v"(voff), "s"(sbase), "s"(lds_base)
+        : "memory");
+}
+
+// LDS-count wait (all LDS ops of this wave done), the others unconstrained
+__device__ __forceinline__ void wait_lgkm0() {
+    __builtin_amdgcn_s_waitcnt(0xF | (0x3 << 14) | (0x7 << 4));
+}
+
+// Workgroup barrier for the ring: every wave has already waited for its own
+// loads into the slot that becomes readable, so no memory wait is needed here
+// (a fenced __syncthreads would drain the loads still in flight for the next
+// slot). The empty asm statements keep the compiler from moving LDS accesses
+// across the barrier.
+__device__ __forceinline__ void ring_barrier() {
+    __asm__ volatile("" ::: "memory");
+    __builtin_amdgcn_s_barrier();
+    __asm__ volatile("" ::: "memory");
+}
+
 // MODE 0: q in {2, 4} (w^e exact in fp16, q a power of two)
 // MODE 1: other powers of two (2-limb W)   MODE 2: any other q (2-limb W)
+//
+// Support blocks stream through four LDS slots, two pairs, filled by
+// global->LDS loads (16 B per lane, lane L at slot base + 16 L): while the
+// MFMAs of one pair run, the other pair's loads are in flight, and no
+// registers hold staged data. The slots are separate arrays and the loop is
+// unrolled by four, so every LDS access names its slot. One wait and one
+// barrier per pair of support blocks.
 template <int MODE, int NKB>
 __device__ __forceinline__ void eval_mfma_body(const qsft_eval_mfma_desc* __restrict__ descs,
                                                const int32_t* __restrict__ pre, int n_inst) {
     constexpr int WL = (MODE == 0) ? 1 : kLimbs;
-    __shared__ __align__(16) v4u bsh[kStage];                     // 16 KiB of B fragments
+    __shared__ __align__(16) v4u bs0[kStage], bs1[kStage], bs2[kStage], bs3[kStage];   // 4 x 16 KiB of B fragments
+    __shared__ __align__(16) v4i ks0[NKB * 64], ks1[NKB * 64], ks2[NKB * 64], ks3[NKB * 64];   // support digits
     __shared__ uint32_t wtab[kLimbs][QSFT_MAX_Q];                 // (Re | Im << 16) fp16 bits per limb
 
     const int lblock = xcd_logical_block();
@@ -191,6 +248,7 @@ __device__ __forceinline__ void eval_mfma_body(const qsft_eval_mfma_desc* __rest
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int g = lane >> 4, r16 = lane & 15;
     const ModQ mq((uint32_t)q);
+    const uint32_t qshift = (q == 2) ? 1u : 0u;                   // MODE 0: exponent of i is E0 (q = 4) or 2 E0
     const int lbase = lb * kRowsPerBlock + wave * 64;
 
     // base-query digit fragments (B operand of the int8 MFMA): column l = lbase + 16 t + r16
@@ -228,57 +286,72 @@ __device__ __forceinline__ void eval_mfma_body(const qsft_eval_mfma_desc* __rest
 
     const v4u* __restrict__ Bg = reinterpret_cast<const v4u*>(d.Bf);
     const int8_t* __restrict__ Kd = d.K;
-    auto frag_base = [&](int sb) { return (((size_t)cb * nsb + sb) * npt + pt0) * kLimbs * 64; };
-    auto load_k = [&](int sb, v4i (&kf)[NKB]) {
-        const int s = 16 * sb + r16;
+    // Loads of one support block: kStagePerThread B fragments per lane (p-tiles
+    // beyond npt_here re-read the last valid fragment: they only feed
+    // accumulators that are never stored), and wave kb < NKB loads digit
+    // block kb of the 16 supports (rows past S re-read row S - 1, whose B rows
+    // are zero). Group size in loads per lane: kStagePerThread (+1 for those waves).
+    auto issue = [&](int sb, v4u* bdst, v4i* kdst) {
+        const v4u* bbase = Bg + (((size_t)cb * nsb + sb) * npt + pt0) * kLimbs * 64;
 #pragma unroll
-        for (int kb = 0; kb < NKB; ++kb) {
-            v4i z = {0, 0, 0, 0};
-            if (s < S) z = *reinterpret_cast<const v4i*>(Kd + (size_t)s * n_pad + 64 * kb + 16 * g);
-            kf[kb] = z;
+        for (int i = 0; i < kStagePerThread; ++i) {
+            const uint32_t e = (uint32_t)min(tid + i * kMfThreads, nstage - 1);
+            dma16(bbase, e * 16u, __builtin_amdgcn_readfirstlane(lds_addr(bdst + i * kMfThreads + wave * 64)));
+        }
+        if (wave < NKB) {
+            const int s = min(16 * sb + r16, S - 1);
+            dma16(Kd, (uint32_t)(s * n_pad + 64 * wave + 16 * g),
+                  __builtin_amdgcn_readfirstlane(lds_addr(kdst + wave * 64)));
+        }
+    };
+    // all but the newest `keep` groups (0..2) complete
+    auto wait_groups = [&](int keep) {
+        if (wave < NKB) {
+            if (keep >= 2) wait_vm<2 * (kStagePerThread + 1)>();
+            else if (keep == 1) wait_vm<kStagePerThread + 1>();
+            else wait_vm<0>();
+        } else {
+            if (keep >= 2) wait_vm<2 * kStagePerThread>();
+            else if (keep == 1) wait_vm<kStagePerThread>();
+            else wait_vm<0>();
         }
     };
 
-    // prologue: support block 0 into registers, then into LDS. p-tiles beyond
-    // npt_here are staged as zeros so the MFMA loop below has no branches.
-    v4u stage[kStagePerThread];
-    v4i kf[NKB];
-    {
-        const size_t fb = frag_base(0);
+    // one support block: W fragments from the int8 MFMA, then the f16 GEMM
+    auto compute = [&](const v4u* bcur, const v4i* kcur) {
+        v4i kf[NKB];
 #pragma unroll
-        for (int i = 0; i < kStagePerThread; ++i) {
-            const int e = tid + i * kMfThreads;
-            stage[i] = (e < nstage) ? Bg[fb + e] : v4u{0u, 0u, 0u, 0u};
-        }
-        load_k(0, kf);
-#pragma unroll
-        for (int i = 0; i < kStagePerThread; ++i) bsh[tid + i * kMfThreads] = stage[i];
-    }
-    __syncthreads();     // fragments of block 0 and wtab visible
-
-    int sb = 0;
-    do {                 // nsb >= 1 (S >= 1 is checked on the host)
-        // W fragments for the 4 subtiles of this support block
-        v4u wa[4][WL];
+        for (int kb = 0; kb < NKB; ++kb) kf[kb] = kcur[kb * 64 + lane];
+        // all four int8 products first, so their latency overlaps
+        v4i e4[4];
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
             v4i e = {0, 0, 0, 0};
 #pragma unroll
             for (int kb = 0; kb < NKB; ++kb) e = __builtin_amdgcn_mfma_i32_16x16x64_i8(kf[kb], bq[t][kb], e, 0, 0, 0);
-            uint32_t r[4];
+            e4[t] = e;
+        }
+        v4u wa[4][WL];
 #pragma unroll
-            for (int v = 0; v < 4; ++v) r[v] = (MODE <= 1) ? ((uint32_t)e[v] & (uint32_t)(q - 1)) : mq((uint32_t)e[v]);
+        for (int t = 0; t < 4; ++t) {
+            const v4i e = e4[t];
             if constexpr (MODE == 0) {
-                // q in {2, 4}: Re w^e = {1, 0, -1, 0}, Im w^e = {0, 1, 0, -1}; for q = 2, e in {0, 1} -> {1, -1}
-                uint32_t wr[4], wi[4];
+                // q in {2, 4}, two supports per 32-bit word (E0 >= 0 and < 2^16, so the halves do not mix).
+                // With f = 2 E0 (q = 2) or E0 (q = 4), w^E0 = i^f: Re = {1, 0, -1, 0}, Im = {0, 1, 0, -1}
+                // over f mod 4, i.e. magnitude fp16 1.0 (0x3C00), sign from bit 1, in Re (bit 0 clear) or Im.
 #pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    const uint32_t ee = (q == 2) ? 2 * r[v] : r[v];
-                    wr[v] = (ee == 0) ? 0x3C00u : (ee == 2 ? 0xBC00u : 0u);   // fp16 +-1
-                    wi[v] = (ee == 1) ? 0x3C00u : (ee == 3 ? 0xBC00u : 0u);
+                for (int h = 0; h < 2; ++h) {
+                    const uint32_t pk = (((uint32_t)e[2 * h + 1] << 16) | ((uint32_t)e[2 * h] & 0xFFFFu)) << qshift;
+                    const uint32_t sgn = ((pk & 0x00020002u) << 14) | 0x3C003C00u;     // +-1.0 in each half
+                    const uint32_t m = (pk & 0x00010001u) * 0xFFFFu;                  // 0xFFFF where Im
+                    const uint32_t im = sgn & m;
+                    wa[t][0][h] = sgn ^ im;                                         // Re words 0, 1
+                    wa[t][0][2 + h] = im;                                           // Im words 2, 3
                 }
-                wa[t][0] = v4u{wr[0] | (wr[1] << 16), wr[2] | (wr[3] << 16), wi[0] | (wi[1] << 16), wi[2] | (wi[3] << 16)};
             } else {
+                uint32_t r[4];
+#pragma unroll
+                for (int v = 0; v < 4; ++v) r[v] = (MODE == 1) ? ((uint32_t)e[v] & (uint32_t)(q - 1)) : mq((uint32_t)e[v]);
 #pragma unroll
                 for (int li = 0; li < WL; ++li) {
                     uint32_t w[4];
@@ -289,26 +362,19 @@ __device__ __forceinline__ void eval_mfma_body(const qsft_eval_mfma_desc* __rest
                 }
             }
         }
-        // prefetch the next support block while the MFMAs below run
-        const bool more = sb + 1 < nsb;
-        if (more) {
-            const size_t fb = frag_base(sb + 1);
-#pragma unroll
-            for (int i = 0; i < kStagePerThread; ++i) {
-                const int e = tid + i * kMfThreads;
-                stage[i] = (e < nstage) ? Bg[fb + e] : v4u{0u, 0u, 0u, 0u};
-            }
-            load_k(sb + 1, kf);
-        }
         // limb-major order: an accumulator is revisited only every 8 x 4 MFMAs
 #pragma unroll
         for (int wl = 0; wl < WL; ++wl) {
 #pragma unroll
             for (int bl = 0; bl < kLimbs; ++bl) {
                 if (wl + bl > 1) continue;                  // drop the lo x lo product (~2^-22 relative)
+                // two fragments in registers: the read of p-tile pt + 1 is in
+                // flight while the MFMAs of p-tile pt run
+                v4u bnext = bcur[bl * 64 + lane];
 #pragma unroll
                 for (int pt = 0; pt < kPtPerBlock; ++pt) {
-                    const v8h bf = __builtin_bit_cast(v8h, bsh[(pt * kLimbs + bl) * 64 + lane]);
+                    const v8h bf = __builtin_bit_cast(v8h, bnext);
+                    if (pt + 1 < kPtPerBlock) bnext = bcur[((pt + 1) * kLimbs + bl) * 64 + lane];
 #pragma unroll
                     for (int t = 0; t < 4; ++t)
                         acc[t][pt] = __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(v8h, wa[t][wl]), bf,
@@ -316,13 +382,35 @@ __device__ __forceinline__ void eval_mfma_body(const qsft_eval_mfma_desc* __rest
                 }
             }
         }
-        __syncthreads();     // every wave is done with this block's fragments
-        if (more) {
-#pragma unroll
-            for (int i = 0; i < kStagePerThread; ++i) bsh[tid + i * kMfThreads] = stage[i];
-        }
-        __syncthreads();     // next block's fragments visible
-    } while (++sb < nsb);
+    };
+    // The four slots form two pairs; a barrier closes each pair of support
+    // blocks. While pair A is computed, pair B's loads are in flight; after
+    // the barrier (every wave is done with pair A) pair A is refilled with the
+    // blocks four ahead.
+    issue(0, bs0, ks0);
+    if (nsb > 1) issue(1, bs1, ks1);
+    if (nsb > 2) issue(2, bs2, ks2);
+    if (nsb > 3) issue(3, bs3, ks3);
+    wait_groups(max(0, min(nsb - 2, 2)));            // blocks 0 and 1 have landed
+    wait_lgkm0();                                    // this wave's wtab writes
+    ring_barrier();                                  // ... for every wave; wtab visible
+
+    for (int sb = 0; sb < nsb; sb += 4) {
+        compute(bs0, ks0);
+        if (sb + 1 < nsb) compute(bs1, ks1);
+        if (sb + 2 >= nsb) break;
+        wait_groups(0);                              // pair B (blocks sb + 2, sb + 3) has landed
+        ring_barrier();                              // ... for every wave; pair A is free
+        if (sb + 4 < nsb) issue(sb + 4, bs0, ks0);
+        if (sb + 5 < nsb) issue(sb + 5, bs1, ks1);
+        compute(bs2, ks2);
+        if (sb + 3 < nsb) compute(bs3, ks3);
+        if (sb + 4 >= nsb) break;
+        wait_groups(0);                              // pair A (blocks sb + 4, sb + 5) has landed
+        ring_barrier();                              // ... for every wave; pair B is free
+        if (sb + 6 < nsb) issue(sb + 6, bs2, ks2);
+        if (sb + 7 < nsb) issue(sb + 7, bs3, ks3);
+    }
 
     // store: lane holds column n = r16 -> delay row p = 8 (pt0 + pt) + n / 2, Re (n even) / Im (n odd),
     // for base queries l = lbase + 16 t + 4 g + v. Pair Re/Im across lanes (xor 1).
