@@ -191,6 +191,9 @@ typedef struct qsft_assemble_desc {
     double sigma;           /* QSFT_NOISE_PHILOX: per-component sd = sqrt(sigma^2 / 2)     */
     uint64_t seed;          /* QSFT_NOISE_PHILOX key                                        */
     double* U;              /* [C][R*p1][Q] (or [C][Q][R*p1]) complex FP64 out              */
+    double* energy;         /* optional [C][Q]: |U column|^2 of every bin as the peeler     *
+                             * computes it (bin-major and R*p1 <= 128 only), for its      *
+                             * first-pass zeroton screen; NULL: not written                */
 } qsft_assemble_desc;
 
 int qsft_assemble_blocks(const qsft_assemble_desc* host_desc);
@@ -232,6 +235,8 @@ typedef struct qsft_peel_desc {
     const int8_t* Mt;       /* [C][b][n_pad]                                                */
     const int8_t* D;        /* [C][P][n_pad]                                                */
     double* U;              /* [C][Q][P] complex FP64 (bin-major), modified in place       */
+    const double* energy;   /* optional [C][Q] first-pass |col|^2 from qsft_assemble (set   *
+                             * after qsft_peel_bind); NULL: the screen reads U             */
     /* workspace, carved by qsft_peel_bind() */
     int8_t* kbin;           /* [C*Q][n_pad]  k of singleton bins of the current pass        */
     double* rho_bin;        /* [C*Q][2]                                                     */

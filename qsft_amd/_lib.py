@@ -52,7 +52,7 @@ class AssembleDesc(ctypes.Structure):
     _fields_ = [("Q", i32), ("C", i32), ("R", i32), ("p1", i32), ("src_rows", i32), ("src_prec", i32),
                 ("noise_mode", i32), ("bin_major", i32),
                 ("gsel", i32 * QSFT_MAX_C), ("rsel", i32 * QSFT_MAX_R),
-                ("src", vp), ("noise", vp), ("sigma", f64), ("seed", u64), ("U", vp)]
+                ("src", vp), ("noise", vp), ("sigma", f64), ("seed", u64), ("U", vp), ("energy", vp)]
 
 
 class PeelDesc(ctypes.Structure):
@@ -60,7 +60,7 @@ class PeelDesc(ctypes.Structure):
                 ("n_pad", i32), ("p1", i32), ("method", i32), ("max_iter", i32),
                 ("hcap_log2", i32), ("out_cap", i32), ("flags", i32),
                 ("cutoff", f64), ("peel_cap", f64),
-                ("Mt", vp), ("D", vp), ("U", vp),
+                ("Mt", vp), ("D", vp), ("U", vp), ("energy", vp),
                 ("kbin", vp), ("rho_bin", vp), ("slot_bin", vp), ("mark", vp), ("sstamp", vp),
                 ("tag", vp), ("owner", vp),
                 ("last", vp), ("head", vp), ("count", vp), ("key", vp), ("rec_time", vp),

@@ -11,7 +11,10 @@
 // are one contiguous run of U. Philox noise is drawn in the write phase, one
 // Philox block for the two elements (c, p, 2h) and (c, p, 2h + 1), so a
 // thread writes two adjacent bins of one row. The noise is a function of the
-// logical element only, so both layouts draw the same values.
+// logical element only, so both layouts draw the same values. With
+// `energy`, the tile keeps the final values and each wave then reduces one
+// bin's column exactly as the peeler does (all rows of a bin are in one tile
+// when R*p1 <= 128).
 #include "qsft_common.h"
 
 namespace qsft {
@@ -116,6 +119,24 @@ __global__ __launch_bounds__(kAsmThreads) void k_assemble(const qsft_assemble_de
         }
         U[((int64_t)c * Q + j) * rows + p] = v0;
         if (jj + 1 < Jc) U[((int64_t)c * Q + j + 1) * rows + p] = v1;
+        if (d.energy) {                          // keep the final values for the energies
+            tile[pr][jj] = v0;
+            if (jj + 1 < Jc) tile[pr][jj + 1] = v1;
+        }
+    }
+    if (!d.energy) return;                       // (rows <= kBmRows here: checked on the host)
+    __syncthreads();
+    // |column|^2 of each of the 16 bins, one wave per bin, rows p = lane + 64 r
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int jj = wave; jj < Jc; jj += kAsmThreads / 64) {
+        double2 u[kColRowsPerLane];
+#pragma unroll
+        for (int r = 0; r < kColRowsPerLane; ++r) {
+            const int p = lane + 64 * r;
+            u[r] = (p < rows) ? tile[p][jj] : make_double2(0.0, 0.0);
+        }
+        const double e = bin_energy(u);
+        if (lane == 0) d.energy[(int64_t)c * Q + j0 + jj] = e;
     }
 }
 
@@ -131,6 +152,7 @@ extern "C" int qsft_assemble_blocks(const qsft_assemble_desc* h) {
         h->src_rows < h->p1 || (h->bin_major != 0 && h->bin_major != 1))
         return -1;
     const long long rows = (long long)h->R * h->p1;
+    if (h->energy && (!h->bin_major || rows > kBmRows)) return -1;   // energies need whole columns per tile
     long long blocks;
     if (h->bin_major) {
         blocks = (long long)h->C * ((h->Q + kBmBins - 1) / kBmBins) * ((rows + kBmRows - 1) / kBmRows);

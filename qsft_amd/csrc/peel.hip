@@ -46,15 +46,10 @@ namespace qsft {
 constexpr int kWaves = 4;             // waves per block of the per-bin kernels
 constexpr int kBinThreads = 64 * kWaves;
 constexpr int kBinsPerWave = 64;      // consecutive bins owned by one wave (one per lane) in the per-bin kernels
-constexpr int kRowsPerLane = 4;       // P <= 256
+constexpr int kRowsPerLane = kColRowsPerLane;   // P <= 256
 constexpr int kSlotThreads = 256;
 constexpr uint32_t kNil = 0xFFFFFFFFu;
 
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
-    return v;
-}
 __device__ __forceinline__ int wave_sum_i(int v) {
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
@@ -139,28 +134,12 @@ __device__ __forceinline__ int digit_weight(int q, int b, int dg) {
     return pw;
 }
 
-// |col|^2 as every zeroton test computes it: per-lane sums over the lane's
-// rows in order with explicit FMAs, an xor butterfly, then sqrt and square
-// (np.linalg.norm(col) ** 2). The screen and kernel A share it, so they
-// classify a column identically.
-__device__ __forceinline__ double bin_energy(const double2 (&u)[kRowsPerLane]) {
-    double sre = 0.0, sim = 0.0;
-#pragma unroll
-    for (int r = 0; r < kRowsPerLane; ++r) {
-        sre = __fma_rn(u[r].x, u[r].x, sre);
-        sim = __fma_rn(u[r].y, u[r].y, sim);
-    }
-    sre = wave_sum(sre);
-    sim = wave_sum(sim);
-    const double e = sqrt(__dadd_rn(sre, sim));
-    return __dmul_rn(e, e);
-}
-
 // First-pass screen. Wave w owns bins [64 w, 64 w + 64); lane L stamps bin
-// 64 w + L for pass 0 iff it is not a zeroton, and -1 (never) otherwise. Four
+// 64 w + L for pass 0 iff it is not a zeroton, and -1 (never) otherwise. With
+// energies from the assembler it compares one number per bin; otherwise four
 // columns are loaded before their energies are reduced, so several loads per
-// lane are in flight: this kernel streams U once at HBM speed, and kernel A
-// then visits only the stamped bins.
+// lane are in flight and U streams once at HBM speed. Kernel A then visits
+// only the stamped bins.
 __global__ __launch_bounds__(kBinThreads) void k_peel_screen(const qsft_peel_desc* __restrict__ descs,
                                                              const int32_t* __restrict__ pre, int n_inst) {
     const int inst = find_inst(pre, n_inst, blockIdx.x);
@@ -171,6 +150,10 @@ __global__ __launch_bounds__(kBinThreads) void k_peel_screen(const qsft_peel_des
     if (base >= CQ) return;
     const int nb = min(kBinsPerWave, CQ - base);
     const double thr = d.cutoff * (double)P;
+    if (d.energy) {
+        if (base + lane < CQ) d.mark[base + lane] = (d.energy[base + lane] > thr) ? 0 : -1;
+        return;
+    }
     const double2* __restrict__ Ub = reinterpret_cast<const double2*>(d.U) + (size_t)base * P;
     bool nz = false;
     for (int g = 0; g < nb; g += 4) {
@@ -731,6 +714,7 @@ extern "C" int qsft_peel_bind(qsft_peel_desc* h, int q, int n, int b, int C, int
     d.mark = reinterpret_cast<int32_t*>(base + lay.off[16]);
     d.sstamp = reinterpret_cast<int32_t*>(base + lay.off[17]);
     d.flags = 0;
+    d.energy = nullptr;
     *h = d;
     return QSFT_OK;
 }

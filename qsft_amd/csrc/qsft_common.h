@@ -90,6 +90,33 @@ __device__ __forceinline__ float4 normal4(uint64_t counter, uint64_t seed) {
                        rb * __builtin_amdgcn_cosf(u2b), rb * __builtin_amdgcn_sinf(u2b));
 }
 
+// xor-butterfly sum over the 64 lanes of a wave; every lane gets the same bits
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
+// Rows per lane of a bin column held by one wave: row p = lane + 64 r (P <= 256).
+constexpr int kColRowsPerLane = 4;
+
+// |col|^2 as every zeroton test computes it: per-lane sums over the lane's
+// rows in order with explicit FMAs, an xor butterfly, then sqrt and square
+// (np.linalg.norm(col) ** 2). The assembler's energies, the peeler's screen
+// and its per-pass detection all call this, so they classify a column alike.
+__device__ __forceinline__ double bin_energy(const double2 (&u)[kColRowsPerLane]) {
+    double sre = 0.0, sim = 0.0;
+#pragma unroll
+    for (int r = 0; r < kColRowsPerLane; ++r) {
+        sre = __fma_rn(u[r].x, u[r].x, sre);
+        sim = __fma_rn(u[r].y, u[r].y, sim);
+    }
+    sre = wave_sum(sre);
+    sim = wave_sum(sim);
+    const double e = sqrt(__dadd_rn(sre, sim));
+    return __dmul_rn(e, e);
+}
+
 // 64-bit mix of a digit vector (FNV-1a over the bytes, then the splitmix64
 // finaliser); equal vectors give equal hashes.
 __device__ __forceinline__ uint64_t mix64(uint64_t z) {

@@ -66,7 +66,7 @@ class TransformSpec:
         cells = self.C * self.P * self.Q
         ws = L.lib.qsft_peel_workspace_bytes(self.q, self.n, self.b, self.C, self.P, 15, 0)
         n_pad = dv.n_pad_of(self.n)
-        return cells * (el + 16) + ws + (self.S + self.C * (self.b + self.P)) * n_pad + self.S * el
+        return cells * (el + 16) + ws + (self.S + self.C * (self.b + self.P)) * n_pad + self.S * el + self.C * self.Q * 8
 
 
 def random_spec(q, n, S, b, C=3, R=2, snr_db=10.0, rng=None, seed=0, distinct=True, method="nso"):
@@ -99,9 +99,11 @@ def random_spec(q, n, S, b, C=3, R=2, snr_db=10.0, rng=None, seed=0, distinct=Tr
 class Batch:
     """eval_mode: "mfma" (matrix-core GEMM form, FP32 only; the default for
     FP32) or "direct" (per-query int8 MFMA phase + VALU w-table accumulation,
-    FP32 or FP64; the plain path)."""
+    FP32 or FP64; the plain path). fused_screen: the assembler also writes each
+    bin's first-pass energy (transforms with P <= 128), so the peeler's
+    zeroton screen does not read U again."""
 
-    def __init__(self, specs, precision="fp32", max_iter=15, device=None, eval_mode=None):
+    def __init__(self, specs, precision="fp32", max_iter=15, device=None, eval_mode=None, fused_screen=True):
         self.specs = list(specs)
         self.precision = precision
         if eval_mode is None:
@@ -123,33 +125,40 @@ class Batch:
             a = dv.strengths_tensor(sp.strengths, "fp64" if eval_mode == "mfma" else precision, dev)
             x = torch.empty((C, P, Q), dtype=dt, device=dev)
             U = torch.empty((C, Q, P), dtype=torch.complex128, device=dev)      # bin-major for the peeler
-            self.bufs.append(dict(Mt=Mt, D=Dd, K=Kd, a=a, x=x, U=U))
+            # first-pass column energies come from the assembler when a bin's rows fit one tile
+            energy = torch.empty((C, Q), dtype=torch.float64, device=dev) if (fused_screen and P <= 128) else None
+            self.bufs.append(dict(Mt=Mt, D=Dd, K=Kd, a=a, x=x, U=U, energy=energy))
             ev.append(dict(q=sp.q, n=sp.n, b=sp.b, C=C, P=P, S=sp.S, Mt=Mt, D=Dd, K=Kd, a=a, a64=a, x=x,
                            shared_d=True, amax=float(np.max(np.abs(sp.strengths))) if sp.S else 0.0))
             df.append(dict(q=sp.q, b=sp.b, rows=C * P, inp=x, out=x))
             item = dict(Q=Q, C=C, R=P // (sp.n + 1), p1=sp.n + 1, src=x, src_rows=P,
-                        gsel=list(range(C)), rsel=list(range(P // (sp.n + 1))), U=U, bin_major=True)
+                        gsel=list(range(C)), rsel=list(range(P // (sp.n + 1))), U=U, bin_major=True,
+                        energy=energy)
             if sp.noise_sd > 0:
                 item.update(sigma=float(sp.noise_sd / np.sqrt(Q)), seed=sp.seed)
             asm.append(item)
             pl.append(dict(q=sp.q, n=sp.n, b=sp.b, C=C, P=P, p1=sp.n + 1,
                            method=L.QSFT_DETECT_NSO1 if sp.method == "nso" else L.QSFT_DETECT_NOISELESS,
-                           cutoff=sp.threshold(), Mt=Mt, D=Dd, U=U, shared_d=True))
+                           cutoff=sp.threshold(), Mt=Mt, D=Dd, U=U, shared_d=True, energy=energy))
         self.eval = K.EvalMfma(ev) if eval_mode == "mfma" else K.EvalCosets(ev, code)
         self.dft = K.Dft(df, code)
         self.asm = K.Assemble(asm)
         self.peel = K.PeelBatch(pl, max_iter=max_iter)
 
-    def launch(self, seed_offset=0, host_loop=False, events=None):
+    def launch(self, seed_offset=0, host_loop=False, events=None, marks=None):
         """Enqueue the whole pipeline on the current stream. With `events`
-        (a dict), HIP events bracket each stage for timing."""
+        (a dict), new HIP events bracket each stage for timing; `marks` maps
+        stage names ("start", "eval_prep", "eval", "dft", "assemble", "peel")
+        to pre-created events to record instead (no event creation per launch)."""
         def mark(name):
+            if marks is not None and name in marks:
+                marks[name].record()
             if events is not None:
                 e = torch.cuda.Event(enable_timing=True)
                 e.record()
                 events.setdefault(name, []).append(e)
         mark("start")
-        if events is not None and hasattr(self.eval, "launch_split"):
+        if (events is not None or marks is not None) and hasattr(self.eval, "launch_split"):
             self.eval.launch_split(lambda: mark("eval_prep"))
         else:
             self.eval.launch()

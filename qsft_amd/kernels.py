@@ -295,7 +295,8 @@ def dft(items, precision):
 
 class Assemble:
     """items: dicts Q, C, R, p1, src (tensor), src_rows, gsel, rsel, U (out), and
-    noise (tensor, complex128 [C][R*p1][Q]) or sigma+seed (Philox) or neither."""
+    noise (tensor, complex128 [C][R*p1][Q]) or sigma+seed (Philox) or neither;
+    optional energy (float64 [C][Q] out: per-bin |column|^2, bin-major only)."""
 
     def __init__(self, items):
         descs, blocks = [], []
@@ -321,9 +322,11 @@ class Assemble:
             else:
                 d.noise_mode = L.QSFT_NOISE_NONE
             d.U = _ptr(it["U"])
+            if it.get("energy") is not None:
+                d.energy = _ptr(it["energy"])
             nb = lib.qsft_assemble_blocks(ctypes.byref(d))
             if nb < 0:
-                raise ValueError("bad assemble shape")
+                raise ValueError("bad assemble shape (energies need bin-major U with R*p1 <= 128)")
             descs.append(d)
             blocks.append(nb)
         self.buf, self.dptr, (self.pre,), (self.tot,) = pack(descs, [blocks], items[0]["U"].device)
@@ -348,7 +351,8 @@ class PeelBatch:
 
     items: dicts q, n, b, C, P, p1, method, cutoff, Mt, D, U (complex128,
     bin-major [C][Q][P]), optional shared_d (all groups use the delay rows of
-    group 0; compared on the device when absent).
+    group 0; compared on the device when absent) and energy (float64 [C][Q]
+    first-pass column energies written by Assemble into the same U).
     """
 
     def __init__(self, items, max_iter=15):
@@ -373,6 +377,8 @@ class PeelBatch:
                 Dg = it["D"]
                 shared = all(torch.equal(Dg[0], Dg[c]) for c in range(1, C))
             d.flags = L.QSFT_PEEL_SHARED_D if shared else 0
+            if it.get("energy") is not None:
+                d.energy = _ptr(it["energy"])
             self.descs.append(d)
             self.ws.append(ws)
         self.bins = [lib.qsft_peel_bin_blocks(ctypes.byref(d)) for d in self.descs]

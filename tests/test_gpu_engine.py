@@ -61,3 +61,34 @@ def test_relaunch_gives_fresh_noise_and_same_support_when_noiseless():
     assert common and any(v1[k] != v2[k] for k in common)       # fresh noise per launch
     assert set(v1) == set(v3) and all(abs(v1[k] - v3[k]) <= 1e-12 * abs(v1[k]) for k in v1)   # same seed
     assert set(map(tuple, r1[1][0])) == {tuple(c) for c in sp_clean.locq.T}
+
+
+def test_fused_screen_matches_streaming_screen():
+    """Energies written by the assembler drive the same first-pass screen as
+    re-reading U: same support and values, transform by transform."""
+    rng = np.random.default_rng(8)
+    specs = [E.random_spec(4, 50, 400, 5, 3, 2, 10.0, rng=rng), E.random_spec(3, 20, 100, 4, 3, 2, 20.0, rng=rng),
+             E.random_spec(4, 64, 300, 5, 3, 1, None, rng=rng, method="identity"),
+             E.random_spec(2, 70, 50, 7, 3, 2, 15.0, rng=rng)]          # P = 142 > 128: streaming screen
+    print("fused vs streaming screen", flush=True)
+    fused = E.Batch(specs, precision="fp32", fused_screen=True)
+    plain = E.Batch(specs, precision="fp32", fused_screen=False)
+    assert fused.bufs[0]["energy"] is not None and fused.bufs[3]["energy"] is None
+    fused.launch(seed_offset=3)
+    plain.launch(seed_offset=3)
+    for a, b in zip(fused.results(), plain.results()):
+        _same((a[0], a[1]), (b[0], b[1]), rtol=1e-12)
+
+
+def test_assembler_energies_are_column_norms():
+    rng = np.random.default_rng(9)
+    sp = E.random_spec(4, 30, 200, 4, 3, 2, 12.0, rng=rng)
+    bt = E.Batch([sp], precision="fp32")
+    bt.eval.launch()                                                     # up to the assembler only:
+    bt.dft.launch()                                                      # the peeler changes U
+    bt.asm.launch(4)
+    torch.cuda.synchronize()
+    U = bt.bufs[0]["U"].cpu().numpy()                                   # [C][Q][P]
+    e = bt.bufs[0]["energy"].cpu().numpy()                              # [C][Q]
+    ref = np.linalg.norm(U, axis=2) ** 2
+    np.testing.assert_allclose(e, ref, rtol=1e-13, atol=0)
