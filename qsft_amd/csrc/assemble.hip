@@ -8,13 +8,12 @@
 //
 // Bin-major: a workgroup owns 16 bins x up to 128 rows; it reads the rows'
 // 16-bin segments into an LDS tile, then writes the 16 bins' columns, which
-// are one contiguous run of U. Philox noise is drawn in the write phase, one
-// Philox block for the two elements (c, p, 2h) and (c, p, 2h + 1), so a
-// thread writes two adjacent bins of one row. The noise is a function of the
-// logical element only, so both layouts draw the same values. With
-// `energy`, the tile keeps the final values and each wave then reduces one
-// bin's column exactly as the peeler does (all rows of a bin are in one tile
-// when R*p1 <= 128).
+// are one contiguous run of U, a pair of bins per wave with lanes along the
+// rows. Philox noise is drawn in the write phase, one Philox block for the two
+// elements (c, p, 2h) and (c, p, 2h + 1). The noise is a function of the
+// logical element only, so both layouts draw the same values. With `energy`,
+// each wave reduces its two columns from registers exactly as the peeler does
+// (all rows of a bin are in one tile when R*p1 <= 128).
 #include "qsft_common.h"
 
 namespace qsft {
@@ -52,6 +51,7 @@ __global__ __launch_bounds__(kAsmThreads) void k_assemble(const qsft_assemble_de
                                                           const int32_t* __restrict__ pre, int n_inst,
                                                           uint64_t seed_offset) {
     __shared__ double2 tile[kBmRows][kBmBins + 1];
+    __shared__ int64_t srow[kBmRows];
     const int inst = find_inst(pre, n_inst, blockIdx.x);
     const qsft_assemble_desc& d = descs[inst];
     const int64_t Q = d.Q;
@@ -86,57 +86,84 @@ __global__ __launch_bounds__(kAsmThreads) void k_assemble(const qsft_assemble_de
     const int p0 = pc * kBmRows, Pc = min(kBmRows, rows - p0);
     const int64_t j0 = (int64_t)jt * kBmBins;
     const int Jc = (int)min((int64_t)kBmBins, Q - j0);
-    // read: 16 lanes cover one row's 16-bin segment
-    for (int e = threadIdx.x; e < Pc * kBmBins; e += kAsmThreads) {
+    // source row of each working row of the chunk (one division per row)
+    if (threadIdx.x < Pc) {
+        const int p = p0 + threadIdx.x;
+        const int r = p / d.p1, i = p - r * d.p1;
+        srow[threadIdx.x] = ((int64_t)d.gsel[c] * d.src_rows + (int64_t)d.rsel[r] * d.p1 + i) * Q;
+    }
+    __syncthreads();
+    // read: 16 lanes cover one row's 16-bin segment; all loads of a thread are
+    // issued before any is stored
+    constexpr int kReads = kBmRows * kBmBins / kAsmThreads;       // 8
+    double2 v[kReads];
+#pragma unroll
+    for (int k = 0; k < kReads; ++k) {
+        const int e = threadIdx.x + k * kAsmThreads;
         const int pr = e >> 4, jj = e & 15;
-        if (jj < Jc) {
-            const int p = p0 + pr;
-            const int64_t j = j0 + jj;
-            tile[pr][jj] = add_array_noise(d, src_value(d, c, p, j), ((int64_t)c * rows + p) * Q + j);
+        v[k] = make_double2(0.0, 0.0);
+        if (pr < Pc && jj < Jc) {
+            const int64_t si = srow[pr] + j0 + jj;
+            if (d.src_prec == QSFT_F32) {
+                const float2 f = reinterpret_cast<const float2*>(d.src)[si];
+                v[k] = make_double2(f.x, f.y);
+            } else {
+                v[k] = reinterpret_cast<const double2*>(d.src)[si];
+            }
         }
     }
+#pragma unroll
+    for (int k = 0; k < kReads; ++k) {
+        const int e = threadIdx.x + k * kAsmThreads;
+        const int pr = e >> 4, jj = e & 15;
+        if (pr < Pc && jj < Jc)
+            tile[pr][jj] = add_array_noise(d, v[k], ((int64_t)c * rows + p0 + pr) * Q + j0 + jj);
+    }
     __syncthreads();
-    // write: element pairs (2h, 2h + 1) of one row; consecutive lanes take consecutive rows
+    // write: wave w takes bin pairs (2h, 2h + 1), h = w, w + 4, ...; lane L holds
+    // rows L and L + 64 of the chunk (the peeler's column layout), so each store
+    // is 1 KiB contiguous and the energies reduce straight from registers
     double2* __restrict__ U = reinterpret_cast<double2*>(d.U);
-    const int npairs = (Jc + 1) >> 1;
-    // o / Pc by multiply-high: exact for o < 1024, Pc in [2, 128] (checked exhaustively); Pc = 1 is o itself
-    const uint32_t mP = (uint32_t)(0xFFFFFFFFull / (uint32_t)Pc) + 1u;
-    const bool philox = d.noise_mode == QSFT_NOISE_PHILOX;
-    for (int o = threadIdx.x; o < npairs * Pc; o += kAsmThreads) {
-        const int h = (Pc == 1) ? o : (int)__umulhi((uint32_t)o, mP);
-        const int pr = o - h * Pc;
-        const int jj = 2 * h;
-        const int p = p0 + pr;
-        const int64_t j = j0 + jj;
-        double2 v0 = tile[pr][jj];
-        double2 v1 = (jj + 1 < Jc) ? tile[pr][jj + 1] : make_double2(0.0, 0.0);
-        if (philox) {
-            const float4 z = normal4(noise_counter(c, p, rows, Q, j), seed);
-            v0.x = v0.x + comp_sd * (double)z.x;
-            v0.y = v0.y + comp_sd * (double)z.y;
-            v1.x = v1.x + comp_sd * (double)z.z;
-            v1.y = v1.y + comp_sd * (double)z.w;
-        }
-        U[((int64_t)c * Q + j) * rows + p] = v0;
-        if (jj + 1 < Jc) U[((int64_t)c * Q + j + 1) * rows + p] = v1;
-        if (d.energy) {                          // keep the final values for the energies
-            tile[pr][jj] = v0;
-            if (jj + 1 < Jc) tile[pr][jj + 1] = v1;
-        }
-    }
-    if (!d.energy) return;                       // (rows <= kBmRows here: checked on the host)
-    __syncthreads();
-    // |column|^2 of each of the 16 bins, one wave per bin, rows p = lane + 64 r
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    for (int jj = wave; jj < Jc; jj += kAsmThreads / 64) {
-        double2 u[kColRowsPerLane];
+    const int npairs = (Jc + 1) >> 1;
+    const bool philox = d.noise_mode == QSFT_NOISE_PHILOX;
+    for (int h = wave; h < npairs; h += kAsmThreads / 64) {
+        const int jj = 2 * h;
+        const int64_t j = j0 + jj;
+        const bool two = jj + 1 < Jc;
+        double2 u0[kColRowsPerLane], u1[kColRowsPerLane];
 #pragma unroll
         for (int r = 0; r < kColRowsPerLane; ++r) {
-            const int p = lane + 64 * r;
-            u[r] = (p < rows) ? tile[p][jj] : make_double2(0.0, 0.0);
+            u0[r] = make_double2(0.0, 0.0);
+            u1[r] = make_double2(0.0, 0.0);
         }
-        const double e = bin_energy(u);
-        if (lane == 0) d.energy[(int64_t)c * Q + j0 + jj] = e;
+#pragma unroll
+        for (int r = 0; r < kBmRows / 64; ++r) {
+            const int pr = lane + 64 * r;
+            if (pr < Pc) {
+                const int p = p0 + pr;
+                double2 v0 = tile[pr][jj];
+                double2 v1 = two ? tile[pr][jj + 1] : make_double2(0.0, 0.0);
+                if (philox) {
+                    const float4 z = normal4(noise_counter(c, p, rows, Q, j), seed);
+                    v0.x = v0.x + comp_sd * (double)z.x;
+                    v0.y = v0.y + comp_sd * (double)z.y;
+                    v1.x = v1.x + comp_sd * (double)z.z;
+                    v1.y = v1.y + comp_sd * (double)z.w;
+                }
+                U[((int64_t)c * Q + j) * rows + p] = v0;
+                if (two) U[((int64_t)c * Q + j + 1) * rows + p] = v1;
+                u0[r] = v0;
+                u1[r] = v1;
+            }
+        }
+        if (d.energy) {                          // rows <= kBmRows here (checked on the host)
+            const double e0 = bin_energy(u0), e1 = bin_energy(u1);
+            if (lane == 0) {
+                d.energy[(int64_t)c * Q + j] = e0;
+                if (two) d.energy[(int64_t)c * Q + j + 1] = e1;
+            }
+        }
     }
 }
 

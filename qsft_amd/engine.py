@@ -103,7 +103,8 @@ class Batch:
     bin's first-pass energy (transforms with P <= 128), so the peeler's
     zeroton screen does not read U again."""
 
-    def __init__(self, specs, precision="fp32", max_iter=15, device=None, eval_mode=None, fused_screen=True):
+    def __init__(self, specs, precision="fp32", max_iter=15, device=None, eval_mode=None, fused_screen=True,
+                 host_rows=True):
         self.specs = list(specs)
         self.precision = precision
         if eval_mode is None:
@@ -143,7 +144,10 @@ class Batch:
         self.eval = K.EvalMfma(ev) if eval_mode == "mfma" else K.EvalCosets(ev, code)
         self.dft = K.Dft(df, code)
         self.asm = K.Assemble(asm)
-        self.peel = K.PeelBatch(pl, max_iter=max_iter)
+        # packed rows: at most a few per planted support entry (noise adds some), kept
+        # in pinned host memory that the pack kernels write directly
+        cap = sum(4 * sp.S + 1024 for sp in self.specs) if host_rows else None
+        self.peel = K.PeelBatch(pl, max_iter=max_iter, host_rows=host_rows, cap_rows=cap)
 
     def launch(self, seed_offset=0, host_loop=False, events=None, marks=None):
         """Enqueue the whole pipeline on the current stream. With `events`
@@ -169,15 +173,19 @@ class Batch:
         mark("assemble")
         self.peel.run(host_loop=host_loop)
         mark("peel")
-        self.done = torch.cuda.Event()
+        # one completion event per batch, re-recorded every launch (a copy that
+        # waits on it captured the previous record when it was enqueued)
+        if getattr(self, "done", None) is None:
+            self.done = torch.cuda.Event()
         self.done.record()
 
     def results(self, with_info=False):
         return self.peel.results(with_info=with_info)
 
-    def fetch(self, copy_stream):
-        """Packed rows of the last launch, copied on `copy_stream` once that launch
-        is done: (counts, offsets, rows). Work queued after it keeps running."""
+    def fetch(self, copy_stream=None):
+        """Packed rows of the last launch once it is done: (counts, offsets, rows).
+        Work queued after it keeps running. With host_rows the rows are already
+        in pinned host memory; otherwise they are copied on `copy_stream`."""
         return self.peel.fetch_packed(copy_stream=copy_stream, ready=self.done)
 
     def unpack(self, packed, with_info=False):

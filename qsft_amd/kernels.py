@@ -355,9 +355,14 @@ class PeelBatch:
     first-pass column energies written by Assemble into the same U).
     """
 
-    def __init__(self, items, max_iter=15):
+    def __init__(self, items, max_iter=15, host_rows=False, cap_rows=None):
+        """host_rows: the pack kernels write the header and rows straight into
+        pinned host memory (read after the launch's completion event; no copy
+        and no second stream). cap_rows: capacity of the packed rows (default:
+        every hash slot of every instance); overflow is reported as an error."""
         self.items = items
         self.max_iter = max_iter
+        self.host_rows = host_rows
         self.descs = []
         self.ws = []
         dev = items[0]["U"].device
@@ -387,10 +392,14 @@ class PeelBatch:
             self.descs, [self.bins, self.slots], dev)
         self.n_pad_max = max(d.n_pad for d in self.descs)
         self.row_stride = self.n_pad_max + 24
-        self.cap_rows = sum(1 << d.hcap_log2 for d in self.descs)
+        self.cap_rows = sum(1 << d.hcap_log2 for d in self.descs) if cap_rows is None else int(cap_rows)
         self.hdr_bytes = lib.qsft_peel_pack_header_bytes(len(self.descs))
-        self.packed = torch.empty(self.hdr_bytes + self.cap_rows * self.row_stride, dtype=torch.uint8, device=dev)
-        self.hdr_host = torch.empty(self.hdr_bytes, dtype=torch.uint8).pin_memory()
+        size = self.hdr_bytes + self.cap_rows * self.row_stride
+        if host_rows:
+            self.packed = torch.empty(size, dtype=torch.uint8).pin_memory()
+        else:
+            self.packed = torch.empty(size, dtype=torch.uint8, device=dev)
+            self.hdr_host = torch.empty(self.hdr_bytes, dtype=torch.uint8).pin_memory()
 
     def run(self, host_loop=False):
         st = stream_handle()
@@ -416,13 +425,24 @@ class PeelBatch:
         return int(self.ws[i][base + 4 * w: base + 4 * w + 4].view(torch.int32).item())
 
     def fetch_packed(self, copy_stream=None, ready=None):
-        """Copy the packed rows to the host: (counts, offsets, rows uint8 [total][row_stride]).
+        """The packed rows on the host: (counts, offsets, rows uint8 [total][row_stride]).
 
-        With `copy_stream` and `ready` (an event recorded after run()), the
-        copies wait only for that event on the copy stream, so later work
-        queued on the compute stream keeps the GPU busy meanwhile.
+        host_rows: wait for `ready` (an event recorded after run(); else the
+        current stream) and read the pinned buffer. Otherwise copy: with
+        `copy_stream` and `ready` the copies wait only for that event on the
+        copy stream, so later work queued on the compute stream keeps running.
         """
         n = len(self.descs)
+        if self.host_rows:
+            if ready is not None:
+                ready.synchronize()
+            else:
+                torch.cuda.current_stream().synchronize()
+            hdr = self.packed[:self.hdr_bytes].numpy().view(np.int64)
+            self._check_header(hdr, n)
+            nbytes = int(hdr[2 * n]) * self.row_stride
+            rows = self.packed[self.hdr_bytes:self.hdr_bytes + nbytes].numpy().copy()
+            return hdr[:n].copy(), hdr[n:2 * n].copy(), rows.reshape(-1, self.row_stride)
         if copy_stream is None:
             self.hdr_host.copy_(self.packed[:self.hdr_bytes])          # synchronising copy
         else:
@@ -431,10 +451,7 @@ class PeelBatch:
                 self.hdr_host.copy_(self.packed[:self.hdr_bytes], non_blocking=True)
             copy_stream.synchronize()
         hdr = self.hdr_host.numpy().view(np.int64)
-        err = int(hdr[2 * n + 1])
-        if err:
-            raise L.QsftLibraryError(f"peel device error bits {err:#x} (1 table full, 2 tag collision, "
-                                     "4 output full)")
+        self._check_header(hdr, n)
         total = int(hdr[2 * n])
         nbytes = total * self.row_stride
         if getattr(self, "rows_host", None) is None or self.rows_host.numel() < nbytes:
@@ -448,6 +465,13 @@ class PeelBatch:
             copy_stream.synchronize()
         rows = self.rows_host[:nbytes].numpy().copy()
         return hdr[:n].copy(), hdr[n:2 * n].copy(), rows.reshape(total, self.row_stride)
+
+    @staticmethod
+    def _check_header(hdr, n):
+        err = int(hdr[2 * n + 1])
+        if err:
+            raise L.QsftLibraryError(f"peel device error bits {err:#x} (1 table full, 2 tag collision, "
+                                     "4 output full)")
 
     def results(self, with_info=True, packed=None):
         """Per instance: (k int64 [m][n], values complex128 [m], order uint64 [m], info dict),

@@ -92,3 +92,16 @@ def test_assembler_energies_are_column_norms():
     e = bt.bufs[0]["energy"].cpu().numpy()                              # [C][Q]
     ref = np.linalg.norm(U, axis=2) ** 2
     np.testing.assert_allclose(e, ref, rtol=1e-13, atol=0)
+
+
+def test_host_rows_match_device_copy():
+    """Rows packed straight into pinned host memory equal the copied ones."""
+    rng = np.random.default_rng(10)
+    specs = [E.random_spec(4, 30, 150, 4, 3, 2, 15.0, rng=rng), E.random_spec(3, 12, 60, 3, 2, 2, None, rng=rng)]
+    print("host rows vs device copy", flush=True)
+    a = E.Batch(specs, precision="fp32", host_rows=True)
+    b = E.Batch(specs, precision="fp32", host_rows=False)
+    a.launch(seed_offset=5)
+    b.launch(seed_offset=5)
+    for x, y in zip(a.unpack(a.fetch()), b.unpack(b.fetch(torch.cuda.Stream()))):
+        _same((x[0], x[1]), (y[0], y[1]), rtol=0.0)
