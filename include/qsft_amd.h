@@ -213,9 +213,10 @@ int qsft_assemble(const qsft_assemble_desc* dev_descs, const int32_t* dev_pre, i
  * previous pass or were peeled into (an unchanged zeroton stays a zeroton) --
  * classifies zeroton / singleton / multiton (k, rho, bin match,
  * residual) and registers the wave's singletons in a device hash table keyed
- * by k, one lane per singleton; kernel B (same bin partition) peels each
- * distinct k with the rho of its last (group, bin) from its bin in every
- * group. The stop rule (no singletons or
+ * by k, one lane per singleton; kernel B (same bin partition) queues each
+ * distinct k, with the rho of its last (group, bin), on its bin in every
+ * group, and the next pass's kernel A subtracts the queued balls from the
+ * columns it owns before classifying them (no FP64 atomics). The stop rule (no singletons or
  * no multitons, the pass still peels), the q^n ball cap and the iteration
  * cap run on the device: no host round trip. The final kernel forms the
  * append-order running mean of every detection of each k
@@ -238,11 +239,13 @@ typedef struct qsft_peel_desc {
     const double* energy;   /* optional [C][Q] first-pass |col|^2 from qsft_assemble (set   *
                              * after qsft_peel_bind); NULL: the screen reads U             */
     /* workspace, carved by qsft_peel_bind() */
-    int8_t* kbin;           /* [C*Q][n_pad]  k of singleton bins of the current pass        */
-    double* rho_bin;        /* [C*Q][2]                                                     */
+    int8_t* kbin;           /* [2][C*Q][n_pad] k of singleton bins, by pass parity          */
+    double* rho_bin;        /* [2][C*Q][2]     rho of singleton bins, by pass parity        */
     int32_t* slot_bin;      /* [C*Q]         hash slot of a singleton bin, -1 otherwise     */
     int32_t* mark;          /* [C*Q]         pass that scans the bin next (-1: none)         */
     int32_t* sstamp;        /* [C*Q]         pass + 1 in which the bin was a singleton      */
+    int32_t* pend_head;     /* [C*Q]         balls to peel from the bin (list head, -1: none) */
+    int32_t* pend_next;     /* [C*Q][C]      list links; entry = source bin * C + group     */
     uint64_t* tag;          /* [H] 0 = empty                                                */
     uint64_t* owner;        /* [H] first (iteration, bin) of the key                        */
     uint64_t* last;         /* [H] last (iteration, bin) of the key                         */

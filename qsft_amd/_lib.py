@@ -62,6 +62,7 @@ class PeelDesc(ctypes.Structure):
                 ("cutoff", f64), ("peel_cap", f64),
                 ("Mt", vp), ("D", vp), ("U", vp), ("energy", vp),
                 ("kbin", vp), ("rho_bin", vp), ("slot_bin", vp), ("mark", vp), ("sstamp", vp),
+                ("pend_head", vp), ("pend_next", vp),
                 ("tag", vp), ("owner", vp),
                 ("last", vp), ("head", vp), ("count", vp), ("key", vp), ("rec_time", vp),
                 ("rec_rho", vp), ("rec_next", vp), ("ctr", vp),

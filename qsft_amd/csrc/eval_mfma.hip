@@ -153,38 +153,6 @@ __global__ __launch_bounds__(256) void k_eval_mfma_prep(const qsft_eval_mfma_des
     }
 }
 
-// s_waitcnt immediate: vector-memory count <= n, the other counters unconstrained (gfx9 encoding)
-template <int N>
-__device__ __forceinline__ void wait_vm() {
-    static_assert(N >= 0 && N < 64, "vmcnt range");
-    __builtin_amdgcn_s_waitcnt((N & 0xF) | ((N >> 4) << 14) | (0x7 << 4) | (0xF << 8));
-}
-
-// LDS byte address of a __shared__ object
-__device__ __forceinline__ uint32_t lds_addr(const void* p) {
-    return (uint32_t)(uintptr_t)(const __attribute__((address_space(3))) char*)p;
-}
-
-// 16 bytes per lane from global memory straight into LDS: lane L writes
-// lds_base + 16 L (lds_base wave-uniform). Issued from asm so the compiler
-// does not track the LDS write: its own tracking cannot tell the ring slots
-// apart and would wait for these loads before every read of another slot.
-// The instruction writes no VGPR; m0 is read when it issues and is restored
-// right after. Every read of a destination slot follows an explicit wait_vm.
-// The global address is a wave-uniform base (SGPR pair) plus a 32-bit
-// per-lane byte offset.
-__device__ __forceinline__ void dma16(const void* sbase, uint32_t voff, uint32_t lds_base) {
-    uint32_t saved;
-    asm volatile(
-        "s_mov_b32 %0, m0\n\t"
-        "s_mov_b32 m0, %3\n\t"
-        "s_nop 0\n\t"
-        "global_load_lds_dwordx4 %1, %2\n\t"
-        "s_mov_b32 m0, %0"
-        : "=&s"(saved)
-        : "v"(voff), "s"(sbase), "s"(lds_base)
-        : "memory");
-}
 
 // LDS-count wait (all LDS ops of this wave done), the others unconstrained
 __device__ __forceinline__ void wait_lgkm0() {

@@ -68,11 +68,20 @@ __device__ __forceinline__ double rem_2pi(double x) {
     return m;
 }
 
-// <row, k> over nw 4-digit words; k is the wave's LDS copy (broadcast reads).
+// <row, k> over nw 4-digit words (nw a multiple of 16, rows 16-byte aligned);
+// k is the wave's LDS copy (broadcast reads), both read 16 bytes at a time.
 __device__ __forceinline__ uint32_t dot_row(const int8_t* __restrict__ row, const uint32_t* k, int nw) {
-    const int* r = reinterpret_cast<const int*>(row);
+    const int4* __restrict__ r = reinterpret_cast<const int4*>(row);
+    const uint4* k4 = reinterpret_cast<const uint4*>(k);
     int acc = 0;
-    for (int w = 0; w < nw; ++w) acc = __builtin_amdgcn_sdot4(r[w], (int)k[w], acc, false);
+    for (int w = 0; w < (nw >> 2); ++w) {
+        const int4 rv = r[w];
+        const uint4 kv = k4[w];
+        acc = __builtin_amdgcn_sdot4(rv.x, (int)kv.x, acc, false);
+        acc = __builtin_amdgcn_sdot4(rv.y, (int)kv.y, acc, false);
+        acc = __builtin_amdgcn_sdot4(rv.z, (int)kv.z, acc, false);
+        acc = __builtin_amdgcn_sdot4(rv.w, (int)kv.w, acc, false);
+    }
     return (uint32_t)acc;
 }
 
@@ -115,7 +124,10 @@ __global__ __launch_bounds__(kSlotThreads) void k_peel_init(const qsft_peel_desc
         d.head[h] = kNil;
         d.count[h] = 0;
     }
-    if (h < d.C * d.Q) d.sstamp[h] = 0;
+    if (h < d.C * d.Q) {
+        d.sstamp[h] = 0;
+        d.pend_head[h] = -1;
+    }
 }
 
 __device__ __forceinline__ void load_col(double2 (&u)[kRowsPerLane], const double2* __restrict__ col, int P,
@@ -176,6 +188,97 @@ __global__ __launch_bounds__(kBinThreads) void k_peel_screen(const qsft_peel_des
     if (base + lane < CQ) d.mark[base + lane] = nz ? 0 : -1;
 }
 
+// A queued ball: the entry index and, fetched ahead, its k word for this lane
+// (lanes < n_pad / 4), its rho and the next entry of the queue.
+struct Pending {
+    int ent, next;
+    uint32_t kw;
+    double rx, ry;
+};
+
+__device__ __forceinline__ Pending fetch_pending(const qsft_peel_desc& d, int ent, int from, int lane) {
+    Pending pd{ent, -1, 0u, 0.0, 0.0};
+    if (ent < 0) return pd;
+    const int C = d.C, nw = d.n_pad >> 2;
+    const size_t CQ = (size_t)C * d.Q;
+    const int sbin = ent / C;                                      // the ball's detection bin
+    const int8_t* __restrict__ kprev = d.kbin + (size_t)(from & 1) * CQ * d.n_pad;
+    const double* __restrict__ rprev = d.rho_bin + (size_t)(from & 1) * CQ * 2;
+    if (lane < nw) pd.kw = reinterpret_cast<const uint32_t*>(kprev + (size_t)sbin * d.n_pad)[lane];
+    pd.rx = rprev[2 * (size_t)sbin];
+    pd.ry = rprev[2 * (size_t)sbin + 1];
+    pd.next = d.pend_next[ent];
+    return pd;
+}
+
+// Subtract from bin t's column u (rows p = lane + 64 r, held by the wave) every
+// ball queued on it by pass `from` (first entry `pd`, already fetched):
+// rho w^{<D_c row, k>} with the ball's k and rho from that pass's singleton
+// arrays; write the column back and clear the queue.
+__device__ __forceinline__ void apply_pending(const qsft_peel_desc& d, int t, int c, int from, Pending pd,
+                                              double2 (&u)[kRowsPerLane], uint32_t* kk, const double2* tab,
+                                              const ModQ& mq, int lane) {
+    if (pd.ent < 0) return;
+    const int P = d.P, n_pad = d.n_pad, nw = n_pad >> 2;
+    const int8_t* __restrict__ Dc = d.D + (size_t)((d.flags & QSFT_PEEL_SHARED_D) ? 0 : c) * P * n_pad;
+    for (;;) {
+        if (lane < nw) kk[lane] = pd.kw;
+        wave_lds_sync();
+        const double rx = pd.rx, ry = pd.ry;
+        const int nxt = pd.next;
+        if (nxt >= 0) pd = fetch_pending(d, nxt, from, lane);   // a second ball on this bin (rare)
+#pragma unroll
+        for (int r = 0; r < kRowsPerLane; ++r) {
+            const int p = lane + 64 * r;
+            if (p < P) {
+                const double2 wv = tab[mq(dot_row(Dc + (size_t)p * n_pad, kk, nw))];
+                const double sx = __dsub_rn(__dmul_rn(rx, wv.x), __dmul_rn(ry, wv.y));
+                const double sy = __dadd_rn(__dmul_rn(rx, wv.y), __dmul_rn(ry, wv.x));
+                u[r].x = __dsub_rn(u[r].x, sx);
+                u[r].y = __dsub_rn(u[r].y, sy);
+            }
+        }
+        wave_lds_sync();                                           // kk is rewritten next
+        if (nxt < 0) break;
+    }
+    double2* __restrict__ col = reinterpret_cast<double2*>(d.U) + (size_t)t * P;
+#pragma unroll
+    for (int r = 0; r < kRowsPerLane; ++r) {
+        const int p = lane + 64 * r;
+        if (p < P) col[p] = u[r];
+    }
+    if (lane == 0) d.pend_head[t] = -1;                            // the queue is consumed
+}
+
+// After the last pass: apply what its kernel B queued, so U ends as the
+// reference's loop leaves it. Same 64-bin wave partition over the slot grid
+// (which covers at least C*Q threads); waves without a queued bin exit at once.
+__global__ __launch_bounds__(kSlotThreads) void k_peel_flush(const qsft_peel_desc* __restrict__ descs,
+                                                             const int32_t* __restrict__ pre, int n_inst) {
+    __shared__ double2 tab[QSFT_MAX_Q];
+    __shared__ __align__(16) uint32_t kws[kSlotThreads / 64][QSFT_MAX_N / 4];
+    const int inst = find_inst(pre, n_inst, blockIdx.x);
+    const qsft_peel_desc& d = descs[inst];
+    const uint32_t iters = d.ctr[QSFT_CTR_ITERS];
+    if (iters == 0) return;
+    fill_wtable_d(tab, d.q);
+    __syncthreads();
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int CQ = d.C * d.Q, P = d.P;
+    const int base = ((blockIdx.x - pre[inst]) * (kSlotThreads / 64) + wave) * kBinsPerWave;
+    if (base >= CQ) return;
+    uint64_t todo = __ballot(base + lane < CQ && d.pend_head[base + lane] >= 0);
+    const ModQ mq((uint32_t)d.q);
+    while (todo) {
+        const int t = base + __builtin_ctzll(todo);
+        todo &= todo - 1;
+        double2 u[kRowsPerLane];
+        load_col(u, reinterpret_cast<const double2*>(d.U) + (size_t)t * P, P, lane);
+        const Pending pd = fetch_pending(d, d.pend_head[t], (int)iters - 1, lane);
+        apply_pending(d, t, t / d.Q, (int)iters - 1, pd, u, kws[wave], tab, mq, lane);
+    }
+}
+
 // Kernel A. Wave w of an instance owns bins [64 w, 64 w + 64): it classifies
 // the ones this pass scans (mark == pass), prefetching
 // the next bin's column while it works on the current one, and buffers its
@@ -186,8 +289,8 @@ __global__ __launch_bounds__(kBinThreads) void k_peel_screen(const qsft_peel_des
 __global__ __launch_bounds__(kBinThreads) __attribute__((amdgpu_waves_per_eu(3))) void k_peel_detect(const qsft_peel_desc* __restrict__ descs,
                                                              const int32_t* __restrict__ pre, int n_inst, int it) {
     __shared__ double2 tab[QSFT_MAX_Q];
-    __shared__ uint32_t kws[kWaves][QSFT_MAX_N / 4];
-    __shared__ double2 colw[kWaves][64 * kRowsPerLane];
+    __shared__ __align__(16) uint32_t kws[kWaves][QSFT_MAX_N / 4];
+    __shared__ __align__(16) double2 colw[kWaves][2][64 * kRowsPerLane];   // current / next column
     __shared__ uint64_t sb_h[kWaves][kBinsPerWave];
     __shared__ double2 sb_rho[kWaves][kBinsPerWave];
     __shared__ int32_t sb_t[kWaves][kBinsPerWave];
@@ -212,33 +315,66 @@ __global__ __launch_bounds__(kBinThreads) __attribute__((amdgpu_waves_per_eu(3))
 
     uint32_t* kk = kws[wave];
     int8_t* kb8 = reinterpret_cast<int8_t*>(kk);
-    double2* cw = colw[wave];
     const ModQ mq((uint32_t)q);
     const double thr = d.cutoff * (double)P;
     const double twopi = 6.283185307179586;
     const double2* __restrict__ Ub = reinterpret_cast<const double2*>(d.U);
+    int8_t* __restrict__ kcur = d.kbin + (size_t)(it & 1) * CQ * n_pad;   // this pass's k and rho
+    double* __restrict__ rcur = d.rho_bin + (size_t)(it & 1) * CQ * 2;
     int nz = 0, ns = 0;                                            // wave-uniform
 
+    // queue heads of the wave's bins (balls the previous pass peeled into them)
+    const int qhead = (it > 0 && want) ? d.pend_head[base + lane] : -1;
+    // columns stream through two LDS slots per wave (global->LDS loads, no
+    // registers held): the next bin's column lands while this one is classified
+    const int nrow = (P + 63) >> 6;                                // loads per column
+    const uint32_t rowoff = (uint32_t)min(lane, P - 1) * 16u;     // rows past P re-read row P - 1
+    auto issue_col = [&](int tb, double2* slotp) {
+        const double2* cb = Ub + (size_t)tb * P;
+        for (int r = 0; r < nrow; ++r)
+            dma16(cb, min(64 * r * 16u + rowoff, (uint32_t)(P - 1) * 16u),
+                  __builtin_amdgcn_readfirstlane(lds_addr(slotp + 64 * r)));
+    };
+    auto wait_cols = [&](bool keep_next) {                       // all but the next column's loads done
+        if (!keep_next) wait_vm<0>();
+        else if (nrow == 1) wait_vm<1>();
+        else if (nrow == 2) wait_vm<2>();
+        else if (nrow == 3) wait_vm<3>();
+        else wait_vm<4>();
+        __asm__ volatile("" ::: "memory");
+    };
     int cur = __builtin_ctzll(todo);
-    double2 un[kRowsPerLane];
-    load_col(un, Ub + (size_t)(base + cur) * P, P, lane);
+    int cslot = 0;
+    issue_col(base + cur, colw[wave][0]);
+    Pending pn = fetch_pending(d, __shfl(qhead, cur), it - 1, lane);
     for (;;) {
         const int t = base + cur;
-        double2 u[kRowsPerLane];
-#pragma unroll
-        for (int r = 0; r < kRowsPerLane; ++r) u[r] = un[r];
+        double2* cw = colw[wave][cslot];
+        const Pending pd = pn;
         todo &= todo - 1;
         const bool more = todo != 0;
         const int nxt = more ? __builtin_ctzll(todo) : 0;
-        if (more) load_col(un, Ub + (size_t)(base + nxt) * P, P, lane);
+        if (more) issue_col(base + nxt, colw[wave][cslot ^ 1]);
+        wait_cols(more);                                          // this bin's column is in cw
+        double2 u[kRowsPerLane];
+#pragma unroll
+        for (int r = 0; r < kRowsPerLane; ++r) {
+            const int p = lane + 64 * r;
+            u[r] = (p < P) ? cw[p] : make_double2(0.0, 0.0);
+        }
+        if (more) pn = fetch_pending(d, __shfl(qhead, nxt), it - 1, lane);   // next bin's first queued ball
 
+        const int c = t / Q, j = t - c * Q;
+        // balls queued on this bin by the previous pass
+        apply_pending(d, t, c, it - 1, pd, u, kk, tab, mq, lane);
         if (bin_energy(u) > thr) {                                 // not a zeroton (wave-uniform)
             ++nz;
-            const int c = t / Q, j = t - c * Q;
+            if (pd.ent >= 0) {                                     // the column changed: refresh the LDS copy
 #pragma unroll
-            for (int r = 0; r < kRowsPerLane; ++r) {
-                const int p = lane + 64 * r;
-                if (p < P) cw[p] = u[r];
+                for (int r = 0; r < kRowsPerLane; ++r) {
+                    const int p = lane + 64 * r;
+                    if (p < P) cw[p] = u[r];
+                }
             }
             if (lane < nw) kk[lane] = 0;
             wave_lds_sync();
@@ -285,7 +421,13 @@ __global__ __launch_bounds__(kBinThreads) __attribute__((amdgpu_waves_per_eu(3))
             }
             wave_lds_sync();
 
-            // --- signature, rho, residual (rows across lanes)
+            // --- signature, rho, residual (rows across lanes); the column is
+            // re-read from LDS so its registers are free during the atan2 above
+#pragma unroll
+            for (int r = 0; r < kRowsPerLane; ++r) {
+                const int p = lane + 64 * r;
+                u[r] = (p < P) ? cw[p] : make_double2(0.0, 0.0);
+            }
             const int8_t* __restrict__ Dc = d.D + (size_t)c * P * n_pad;
             double2 w[kRowsPerLane];
             double rr = 0.0, ri = 0.0;
@@ -330,10 +472,10 @@ __global__ __launch_bounds__(kBinThreads) __attribute__((amdgpu_waves_per_eu(3))
 
             if (lane == 0) d.mark[t] = it + 1;                      // non-zeroton: rescanned next pass
             if (single) {
-                if (lane < nw) reinterpret_cast<uint32_t*>(d.kbin + (size_t)t * n_pad)[lane] = kk[lane];
+                if (lane < nw) reinterpret_cast<uint32_t*>(kcur + (size_t)t * n_pad)[lane] = kk[lane];
                 if (lane == 0) {
-                    d.rho_bin[2 * (size_t)t] = rho_x;
-                    d.rho_bin[2 * (size_t)t + 1] = rho_y;
+                    rcur[2 * (size_t)t] = rho_x;
+                    rcur[2 * (size_t)t + 1] = rho_y;
                     sb_t[wave][ns] = t;
                     sb_h[wave][ns] = hash_words(kk, nw);
                     sb_rho[wave][ns] = make_double2(rho_x, rho_y);
@@ -344,6 +486,8 @@ __global__ __launch_bounds__(kBinThreads) __attribute__((amdgpu_waves_per_eu(3))
         }
         if (!more) break;
         cur = nxt;
+        cslot ^= 1;
+        wave_lds_sync();                                           // this slot's LDS writes done before it is refilled
     }
 
     if (lane == 0) {
@@ -406,15 +550,14 @@ __global__ __launch_bounds__(kBinThreads) __attribute__((amdgpu_waves_per_eu(3))
 }
 
 // Kernel B. Same bin partition as kernel A; the wave walks the singletons of
-// this pass among its 16 bins (sstamp == pass + 1). For each: the k must equal
-// its slot's key; the last detection of the key peels it from every group
-// (one target bin per group, C x b bin digits computed by as many lanes at
-// once) with FP64 atomics over the target's contiguous column, and stamps the
-// target for the next pass's scan.
+// this pass among its 64 bins (sstamp == pass + 1). For each: the k must equal
+// its slot's key; the last detection of the key queues the ball on its bin in
+// every group (one target per group, C x b bin digits computed by as many
+// lanes at once; one atomicExch per target) and stamps the target for the
+// next pass, whose kernel A subtracts it from the column it owns.
 __global__ __launch_bounds__(kBinThreads) void k_peel_apply(const qsft_peel_desc* __restrict__ descs,
                                                             const int32_t* __restrict__ pre, int n_inst, int it) {
-    __shared__ double2 tab[QSFT_MAX_Q];
-    __shared__ uint32_t kws[kWaves][QSFT_MAX_N / 4];
+    __shared__ __align__(16) uint32_t kws[kWaves][QSFT_MAX_N / 4];
     const int inst = find_inst(pre, n_inst, blockIdx.x);
     const qsft_peel_desc& d = descs[inst];
     uint32_t* __restrict__ ctr = d.ctr;
@@ -436,10 +579,8 @@ __global__ __launch_bounds__(kBinThreads) void k_peel_apply(const qsft_peel_desc
         if (it + 1 < 16) ctr[(it + 1) * QSFT_CTR_ITER_STRIDE + 3] = next;
     }
     if (!active) return;
-    const int q = d.q, b = d.b, C = d.C, P = d.P, Q = d.Q, n_pad = d.n_pad;
+    const int q = d.q, b = d.b, C = d.C, Q = d.Q, n_pad = d.n_pad;
     const int nw = n_pad >> 2;
-    fill_wtable_d(tab, q);
-    __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int CQ = C * Q;
     const int base = (local * kWaves + wave) * kBinsPerWave;
@@ -449,15 +590,15 @@ __global__ __launch_bounds__(kBinThreads) void k_peel_apply(const qsft_peel_desc
     if (todo == 0) return;
     uint32_t* kk = kws[wave];
     const ModQ mq((uint32_t)q);
-    const bool shared_d = (d.flags & QSFT_PEEL_SHARED_D) != 0;
     const bool all_digits = C * b <= 64;          // every group's bin digits in one step
+    const int8_t* __restrict__ kcur = d.kbin + (size_t)(it & 1) * CQ * n_pad;
 
     while (todo) {
         const int t = base + __builtin_ctzll(todo);
         todo &= todo - 1;
         const int slot = d.slot_bin[t];
         const uint64_t tm = ((uint64_t)(it + 1) << 32) | (uint32_t)t;
-        const uint32_t* ksrc = reinterpret_cast<const uint32_t*>(d.kbin + (size_t)t * n_pad);
+        const uint32_t* ksrc = reinterpret_cast<const uint32_t*>(kcur + (size_t)t * n_pad);
         if (lane < nw) kk[lane] = ksrc[lane];
         wave_lds_sync();
         // the detection's k must equal its slot's key
@@ -468,54 +609,45 @@ __global__ __launch_bounds__(kBinThreads) void k_peel_apply(const qsft_peel_desc
             if (own == tm) {
                 if (lane < nw) kslot[lane] = kk[lane];
             } else if (lane < nw) {
-                same = reinterpret_cast<const uint32_t*>(d.kbin + (size_t)(uint32_t)own * n_pad)[lane] == kk[lane];
+                same = reinterpret_cast<const uint32_t*>(kcur + (size_t)(uint32_t)own * n_pad)[lane] == kk[lane];
             }
         } else if (lane < nw) {
             same = kslot[lane] == kk[lane];
         }
         if (!__all(same) && lane == 0) atomicOr(&ctr[QSFT_CTR_ERROR], (uint32_t)QSFT_DEVERR_TAG_COLLISION);
         if (d.last[slot] == tm) {
-            // the last detection of this key in this pass peels it from every group
-            const double rx = d.rho_bin[2 * (size_t)t], ry = d.rho_bin[2 * (size_t)t + 1];
-            int hd = 0;
-            if (all_digits && lane < C * b) {
-                const int l = lane / b, dg = lane - l * b;
-                hd = (int)mq(dot_row(d.Mt + ((size_t)l * b + dg) * n_pad, kk, nw)) * digit_weight(q, b, dg);
-            }
-            double sx[kRowsPerLane], sy[kRowsPerLane];
-            for (int l = 0; l < C; ++l) {
-                int part;
-                if (all_digits) {
-                    part = (lane >= l * b && lane < (l + 1) * b) ? hd : 0;
-                } else {
-                    part = (lane < b) ? (int)mq(dot_row(d.Mt + ((size_t)l * b + lane) * n_pad, kk, nw)) *
-                                            digit_weight(q, b, lane)
-                                      : 0;
+            // the last detection of this key in this pass queues it on every group
+            if (all_digits) {
+                int hd = 0;
+                if (lane < C * b) {
+                    const int l = lane / b, dg = lane - l * b;
+                    hd = (int)mq(dot_row(d.Mt + ((size_t)l * b + dg) * n_pad, kk, nw)) * digit_weight(q, b, dg);
                 }
-                const int jl = wave_sum_i(part);
-                if (l == 0 || !shared_d) {
-                    const int8_t* __restrict__ Dl = d.D + (size_t)l * P * n_pad;
-#pragma unroll
-                    for (int r = 0; r < kRowsPerLane; ++r) {
-                        const int p = lane + 64 * r;
-                        sx[r] = sy[r] = 0.0;
-                        if (p < P) {
-                            const double2 wv = tab[mq(dot_row(Dl + (size_t)p * n_pad, kk, nw))];
-                            sx[r] = __dsub_rn(__dmul_rn(rx, wv.x), __dmul_rn(ry, wv.y));
-                            sy[r] = __dadd_rn(__dmul_rn(rx, wv.y), __dmul_rn(ry, wv.x));
-                        }
+                // lane l < C sums the digits of group l (lanes l b .. l b + b - 1)
+                int jl = 0;
+                for (int dg = 0; dg < b; ++dg) {
+                    const int v = __shfl(hd, (lane < C ? lane : 0) * b + dg);
+                    jl += v;
+                }
+                if (lane < C) {
+                    const int target = lane * Q + jl;
+                    const int e = t * C + lane;
+                    d.pend_next[e] = atomicExch(&d.pend_head[target], e);
+                    d.mark[target] = it + 1;                        // the peeled bin is rescanned
+                }
+            } else {
+                for (int l = 0; l < C; ++l) {
+                    const int part = (lane < b) ? (int)mq(dot_row(d.Mt + ((size_t)l * b + lane) * n_pad, kk, nw)) *
+                                                      digit_weight(q, b, lane)
+                                                : 0;
+                    const int jl = wave_sum_i(part);
+                    if (lane == 0) {
+                        const int target = l * Q + jl;
+                        const int e = t * C + l;
+                        d.pend_next[e] = atomicExch(&d.pend_head[target], e);
+                        d.mark[target] = it + 1;
                     }
                 }
-                double* __restrict__ Ucol = d.U + 2 * ((size_t)l * Q + jl) * P;
-#pragma unroll
-                for (int r = 0; r < kRowsPerLane; ++r) {
-                    const int p = lane + 64 * r;
-                    if (p < P) {
-                        unsafeAtomicAdd(Ucol + 2 * p, -sx[r]);
-                        unsafeAtomicAdd(Ucol + 2 * p + 1, -sy[r]);
-                    }
-                }
-                if (lane == 0) d.mark[l * Q + jl] = it + 1;        // the peeled bin is rescanned
             }
         }
         wave_lds_sync();                                           // kk is rewritten by the next singleton
@@ -631,12 +763,12 @@ struct PeelLayout {
     size_t off[20];
     size_t total;
 };
-PeelLayout peel_layout(long long CQ, int n_pad, int hl2, int max_iter, int out_cap) {
+PeelLayout peel_layout(long long CQ, int C, int n_pad, int hl2, int max_iter, int out_cap) {
     const size_t H = (size_t)1 << hl2;
     const size_t L = (size_t)max_iter * CQ;
     const size_t sizes[] = {
-        (size_t)CQ * n_pad,      // 0 kbin
-        (size_t)CQ * 16,         // 1 rho_bin
+        (size_t)2 * CQ * n_pad,  // 0 kbin (two pass parities)
+        (size_t)2 * CQ * 16,     // 1 rho_bin
         (size_t)CQ * 4,          // 2 slot_bin
         H * 8, H * 8, H * 8,     // 3-5 tag owner last
         H * 4, H * 4,            // 6-7 head count
@@ -645,6 +777,7 @@ PeelLayout peel_layout(long long CQ, int n_pad, int hl2, int max_iter, int out_c
         QSFT_CTR_WORDS * 4,      // 12 ctr
         (size_t)out_cap * n_pad, (size_t)out_cap * 16, (size_t)out_cap * 8,  // 13-15 outputs
         (size_t)CQ * 4, (size_t)CQ * 4,                                      // 16-17 mark sstamp
+        (size_t)CQ * 4, (size_t)CQ * 4 * C,                                  // 18-19 pend_head pend_next
     };
     PeelLayout lay{};
     size_t o = 0;
@@ -664,7 +797,7 @@ extern "C" size_t qsft_peel_workspace_bytes(int q, int n, int b, int C, int P, i
     const long long CQ = (long long)C * ipow(q, b);
     const int n_pad = (n + 63) / 64 * 64;
     const int hl2 = hcap_log2_for(CQ);
-    return peel_layout(CQ, n_pad, hl2, max_iter, out_cap > 0 ? out_cap : (1 << hl2)).total;
+    return peel_layout(CQ, C, n_pad, hl2, max_iter, out_cap > 0 ? out_cap : (1 << hl2)).total;
 }
 
 extern "C" int qsft_peel_bind(qsft_peel_desc* h, int q, int n, int b, int C, int P, int p1, int method,
@@ -683,7 +816,7 @@ extern "C" int qsft_peel_bind(qsft_peel_desc* h, int q, int n, int b, int C, int
     if (CQ > 0x3FFFFFFF) { qsft_set_error("qsft_peel_bind: too many bins"); return QSFT_ERR_ARG; }
     const int hl2 = hcap_log2_for(CQ);
     if (out_cap <= 0) out_cap = 1 << hl2;
-    const PeelLayout lay = peel_layout(CQ, n_pad, hl2, max_iter, out_cap);
+    const PeelLayout lay = peel_layout(CQ, C, n_pad, hl2, max_iter, out_cap);
     if (ws_bytes < lay.total) { qsft_set_error("qsft_peel_bind: workspace too small"); return QSFT_ERR_ARG; }
     char* base = static_cast<char*>(ws);
     qsft_peel_desc d{};
@@ -713,6 +846,8 @@ extern "C" int qsft_peel_bind(qsft_peel_desc* h, int q, int n, int b, int C, int
     d.out_order = reinterpret_cast<uint64_t*>(base + lay.off[15]);
     d.mark = reinterpret_cast<int32_t*>(base + lay.off[16]);
     d.sstamp = reinterpret_cast<int32_t*>(base + lay.off[17]);
+    d.pend_head = reinterpret_cast<int32_t*>(base + lay.off[18]);
+    d.pend_next = reinterpret_cast<int32_t*>(base + lay.off[19]);
     d.flags = 0;
     d.energy = nullptr;
     *h = d;
@@ -764,6 +899,9 @@ extern "C" int qsft_peel_iteration(const qsft_peel_desc* d, const int32_t* pre_b
 extern "C" int qsft_peel_finalize(const qsft_peel_desc* d, const int32_t* pre_slots, int n_inst, int blocks_slots,
                                   void* stream) {
     if (!d || !pre_slots || n_inst < 1 || blocks_slots < 1) { qsft_set_error("qsft_peel_finalize: bad arguments"); return QSFT_ERR_ARG; }
+    hipLaunchKernelGGL(k_peel_flush, dim3(blocks_slots), dim3(kSlotThreads), 0, (hipStream_t)stream, d, pre_slots, n_inst);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return qsft_hip_status(e, "qsft_peel_finalize(flush)");
     hipLaunchKernelGGL(k_peel_finalize, dim3(blocks_slots), dim3(kSlotThreads), 0, (hipStream_t)stream, d, pre_slots, n_inst);
     return qsft_hip_status(hipGetLastError(), "qsft_peel_finalize");
 }

@@ -117,6 +117,47 @@ __device__ __forceinline__ double bin_energy(const double2 (&u)[kColRowsPerLane]
     return __dmul_rn(e, e);
 }
 
+// LDS byte address of a __shared__ object
+__device__ __forceinline__ uint32_t lds_addr(const void* p) {
+    return (uint32_t)(uintptr_t)(const __attribute__((address_space(3))) char*)p;
+}
+
+// 16 bytes per lane from global memory straight into LDS: lane L writes
+// lds_base + 16 L (lds_base wave-uniform). Issued from asm so the compiler
+// does not track the LDS write: its own tracking cannot tell separate LDS
+// buffers apart and would wait for these loads before every read of another
+// buffer.
+// The instruction writes no VGPR; m0 is read when it issues and is restored
+// right after. Every read of a destination slot follows an explicit wait_vm.
+// The global address is a wave-uniform base (SGPR pair) plus a 32-bit
+// per-lane byte offset.
+__device__ __forceinline__ void dma16(const void* sbase, uint32_t voff, uint32_t lds_base) {
+    // the base is wave-uniform by contract; readfirstlane puts it in SGPRs. It
+    // returns int: widen through uint32_t, or a low word >= 2^31 sign-extends
+    // into the high word and the address is garbage.
+    const uint64_t bv = (uint64_t)(uintptr_t)sbase;
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(bv >> 32));
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)bv);
+    const uint64_t bs = ((uint64_t)hi << 32) | (uint64_t)lo;
+    uint32_t saved;
+    asm volatile(
+        "s_mov_b32 %0, m0\n\t"
+        "s_mov_b32 m0, %3\n\t"
+        "s_nop 0\n\t"
+        "global_load_lds_dwordx4 %1, %2\n\t"
+        "s_mov_b32 m0, %0"
+        : "=&s"(saved)
+        : "v"(voff), "s"(bs), "s"(lds_base)
+        : "memory");
+}
+
+// s_waitcnt immediate: vector-memory count <= n, the other counters unconstrained (gfx9 encoding)
+template <int N>
+__device__ __forceinline__ void wait_vm() {
+    static_assert(N >= 0 && N < 64, "vmcnt range");
+    __builtin_amdgcn_s_waitcnt((N & 0xF) | ((N >> 4) << 14) | (0x7 << 4) | (0xF << 8));
+}
+
 // 64-bit mix of a digit vector (FNV-1a over the bytes, then the splitmix64
 // finaliser); equal vectors give equal hashes.
 __device__ __forceinline__ uint64_t mix64(uint64_t z) {
