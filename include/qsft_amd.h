@@ -264,6 +264,11 @@ typedef struct qsft_peel_desc {
 
 /* qsft_peel_desc::flags */
 #define QSFT_PEEL_SHARED_D 1    /* every group uses the delay rows of group 0            */
+#define QSFT_PEEL_UNIT_D 2      /* delay row i of every repeat block r is row 0 of the   *
+                                 * block + s e_(i-1) (mod q): the NSO delays (s = -1)     *
+                                 * and the identity delays (s = +1); signatures then take *
+                                 * R dot products per key instead of P                   */
+#define QSFT_PEEL_UNIT_D_NEG 4  /* with QSFT_PEEL_UNIT_D: s = -1                          */
 
 /* counter layout (uint32 words); per iteration i at 8 i + :
  *   0 singletons, 1 multitons, 2 distinct balls, 3 active flag,

@@ -22,7 +22,7 @@ QSFT_NOISE_NONE, QSFT_NOISE_ARRAY, QSFT_NOISE_PHILOX = 0, 1, 2
 QSFT_DEVERR_TABLE_FULL, QSFT_DEVERR_TAG_COLLISION, QSFT_DEVERR_OUT_FULL = 1, 2, 4
 QSFT_MAX_Q, QSFT_MAX_N, QSFT_MAX_B, QSFT_MAX_C, QSFT_MAX_R = 64, 256, 16, 16, 16
 QSFT_DFT_MAX_PASSES = 4
-QSFT_PEEL_SHARED_D = 1
+QSFT_PEEL_SHARED_D, QSFT_PEEL_UNIT_D, QSFT_PEEL_UNIT_D_NEG = 1, 2, 4
 QSFT_CTR_ITER_STRIDE = 8
 QSFT_CTR_LOG, QSFT_CTR_OUT, QSFT_CTR_ERROR, QSFT_CTR_ITERS, QSFT_CTR_PEELED = 128, 129, 130, 131, 132
 QSFT_CTR_WORDS = 136

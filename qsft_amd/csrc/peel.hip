@@ -85,6 +85,48 @@ __device__ __forceinline__ uint32_t dot_row(const int8_t* __restrict__ row, cons
     return (uint32_t)acc;
 }
 
+// The same product with the row in LDS.
+__device__ __forceinline__ uint32_t dot_row_lds(const int8_t* row, const uint32_t* k, int nw) {
+    const uint4* r = reinterpret_cast<const uint4*>(row);
+    const uint4* k4 = reinterpret_cast<const uint4*>(k);
+    int acc = 0;
+    for (int w = 0; w < (nw >> 2); ++w) {
+        const uint4 rv = r[w];
+        const uint4 kv = k4[w];
+        acc = __builtin_amdgcn_sdot4((int)rv.x, (int)kv.x, acc, false);
+        acc = __builtin_amdgcn_sdot4((int)rv.y, (int)kv.y, acc, false);
+        acc = __builtin_amdgcn_sdot4((int)rv.z, (int)kv.z, acc, false);
+        acc = __builtin_amdgcn_sdot4((int)rv.w, (int)kv.w, acc, false);
+    }
+    return (uint32_t)acc;
+}
+
+// Block-level LDS copies of small per-instance tables (filled once per block):
+// the offset rows of structured delays (group 0 when shared) and every
+// group's M^T rows. Larger tables stay in global memory.
+constexpr int kStageOffBytes = 2048;
+constexpr int kStageMtBytes = 4096;
+
+__device__ __forceinline__ const int8_t* stage_offsets(const qsft_peel_desc& d, int8_t* buf) {
+    const int R = d.P / d.p1, n_pad = d.n_pad;
+    if (!(d.flags & QSFT_PEEL_UNIT_D) || !(d.flags & QSFT_PEEL_SHARED_D) || R * n_pad > kStageOffBytes)
+        return nullptr;
+    const int words = R * n_pad / 16;
+    for (int e = threadIdx.x; e < words; e += blockDim.x) {
+        const int r = e / (n_pad / 16), w = e - r * (n_pad / 16);
+        reinterpret_cast<uint4*>(buf)[e] = reinterpret_cast<const uint4*>(d.D + (size_t)r * d.p1 * n_pad)[w];
+    }
+    return buf;
+}
+
+__device__ __forceinline__ const int8_t* stage_mt(const qsft_peel_desc& d, int8_t* buf) {
+    const int bytes = d.C * d.b * d.n_pad;
+    if (bytes > kStageMtBytes) return d.Mt;
+    for (int e = threadIdx.x; e < bytes / 16; e += blockDim.x)
+        reinterpret_cast<uint4*>(buf)[e] = reinterpret_cast<const uint4*>(d.Mt)[e];
+    return buf;
+}
+
 __device__ __forceinline__ uint64_t hash_words(const uint32_t* k, int nw) {
     uint64_t h = 0xCBF29CE484222325ull;
     for (int w = 0; w < nw; ++w) {
@@ -188,6 +230,48 @@ __global__ __launch_bounds__(kBinThreads) void k_peel_screen(const qsft_peel_des
     if (base + lane < CQ) d.mark[base + lane] = nz ? 0 : -1;
 }
 
+// w^{<D_c row p, k>} for this lane's rows p = lane + 64 r (zero past P); k is
+// the wave's LDS copy (words kk, digits as bytes). With QSFT_PEEL_UNIT_D,
+// row p = r p1 + i is offset_r + s e_(i-1), so <row, k> = <offset_r, k> + s k_(i-1)
+// (i > 0): lanes r < R form the R offset products, every row takes one of them
+// and one digit. Otherwise each row is a full dot product.
+__device__ __forceinline__ void row_phases(const qsft_peel_desc& d, const int8_t* __restrict__ Dc,
+                                           const uint32_t* kk, const double2* tab, const ModQ& mq, int lane,
+                                           double2 (&w)[kRowsPerLane], const int8_t* offs = nullptr) {
+    const int P = d.P, n_pad = d.n_pad, nw = n_pad >> 2;
+    if (d.flags & QSFT_PEEL_UNIT_D) {
+        const int p1 = d.p1, R = P / p1, q = d.q;
+        const bool neg = (d.flags & QSFT_PEEL_UNIT_D_NEG) != 0;
+        const int8_t* kb = reinterpret_cast<const int8_t*>(kk);
+        const uint32_t m1 = (uint32_t)(0xFFFFFFFFull / (uint32_t)p1) + 1u;   // p / p1 exact for p < 256
+        int doff = 0;
+        if (lane < R)                                              // offset rows: LDS copy when staged
+            doff = (int)mq(offs ? dot_row_lds(offs + (size_t)lane * n_pad, kk, nw)
+                                : dot_row(Dc + (size_t)lane * p1 * n_pad, kk, nw));
+#pragma unroll
+        for (int r = 0; r < kRowsPerLane; ++r) {
+            const int p = lane + 64 * r;
+            const uint32_t pc = (uint32_t)min(p, P - 1);
+            const int blk = (int)__umulhi(pc, m1);
+            const int i = (int)pc - blk * p1;
+            const int base = __shfl(doff, blk);                    // every lane takes part
+            w[r] = make_double2(0.0, 0.0);
+            if (p < P) {
+                int e = base;
+                if (i > 0) e += neg ? q - (int)kb[i - 1] : (int)kb[i - 1];
+                if (e >= q) e -= q;
+                w[r] = tab[e];
+            }
+        }
+    } else {
+#pragma unroll
+        for (int r = 0; r < kRowsPerLane; ++r) {
+            const int p = lane + 64 * r;
+            w[r] = (p < P) ? tab[mq(dot_row(Dc + (size_t)p * n_pad, kk, nw))] : make_double2(0.0, 0.0);
+        }
+    }
+}
+
 // A queued ball: the entry index and, fetched ahead, its k word for this lane
 // (lanes < n_pad / 4), its rho and the next entry of the queue.
 struct Pending {
@@ -197,6 +281,7 @@ struct Pending {
 };
 
 __device__ __forceinline__ Pending fetch_pending(const qsft_peel_desc& d, int ent, int from, int lane) {
+    ent = __builtin_amdgcn_readfirstlane(ent);                     // wave-uniform: scalar loads below
     Pending pd{ent, -1, 0u, 0.0, 0.0};
     if (ent < 0) return pd;
     const int C = d.C, nw = d.n_pad >> 2;
@@ -214,32 +299,64 @@ __device__ __forceinline__ Pending fetch_pending(const qsft_peel_desc& d, int en
 // Subtract from bin t's column u (rows p = lane + 64 r, held by the wave) every
 // ball queued on it by pass `from` (first entry `pd`, already fetched):
 // rho w^{<D_c row, k>} with the ball's k and rho from that pass's singleton
-// arrays; write the column back and clear the queue.
+// arrays; write the column back and clear the queue. The queue order comes
+// from an atomic race, so the first kSortedPending entries are applied in
+// ascending entry order (the FP64 result does not depend on the race); more
+// balls than that on one bin in one pass follow in queue order.
+constexpr int kSortedPending = 8;
+
+__device__ __forceinline__ void subtract_ball(const qsft_peel_desc& d, const int8_t* __restrict__ Dc,
+                                              const Pending& pd, double2 (&u)[kRowsPerLane], uint32_t* kk,
+                                              const double2* tab, const ModQ& mq, int lane, const int8_t* offs) {
+    const int nw = d.n_pad >> 2, P = d.P;
+    if (lane < nw) kk[lane] = pd.kw;
+    wave_lds_sync();
+    double2 w[kRowsPerLane];
+    row_phases(d, Dc, kk, tab, mq, lane, w, offs);
+#pragma unroll
+    for (int r = 0; r < kRowsPerLane; ++r) {
+        const int p = lane + 64 * r;
+        if (p < P) {
+            const double sx = __dsub_rn(__dmul_rn(pd.rx, w[r].x), __dmul_rn(pd.ry, w[r].y));
+            const double sy = __dadd_rn(__dmul_rn(pd.rx, w[r].y), __dmul_rn(pd.ry, w[r].x));
+            u[r].x = __dsub_rn(u[r].x, sx);
+            u[r].y = __dsub_rn(u[r].y, sy);
+        }
+    }
+    wave_lds_sync();                                               // kk is rewritten next
+}
+
 __device__ __forceinline__ void apply_pending(const qsft_peel_desc& d, int t, int c, int from, Pending pd,
                                               double2 (&u)[kRowsPerLane], uint32_t* kk, const double2* tab,
-                                              const ModQ& mq, int lane) {
+                                              const ModQ& mq, int lane, const int8_t* offs) {
     if (pd.ent < 0) return;
-    const int P = d.P, n_pad = d.n_pad, nw = n_pad >> 2;
+    const int P = d.P, n_pad = d.n_pad;
     const int8_t* __restrict__ Dc = d.D + (size_t)((d.flags & QSFT_PEEL_SHARED_D) ? 0 : c) * P * n_pad;
-    for (;;) {
-        if (lane < nw) kk[lane] = pd.kw;
-        wave_lds_sync();
-        const double rx = pd.rx, ry = pd.ry;
-        const int nxt = pd.next;
-        if (nxt >= 0) pd = fetch_pending(d, nxt, from, lane);   // a second ball on this bin (rare)
-#pragma unroll
-        for (int r = 0; r < kRowsPerLane; ++r) {
-            const int p = lane + 64 * r;
-            if (p < P) {
-                const double2 wv = tab[mq(dot_row(Dc + (size_t)p * n_pad, kk, nw))];
-                const double sx = __dsub_rn(__dmul_rn(rx, wv.x), __dmul_rn(ry, wv.y));
-                const double sy = __dadd_rn(__dmul_rn(rx, wv.y), __dmul_rn(ry, wv.x));
-                u[r].x = __dsub_rn(u[r].x, sx);
-                u[r].y = __dsub_rn(u[r].y, sy);
-            }
+    // lane a holds the a-th queued entry (the first kSortedPending of them)
+    int mine = (lane == 0) ? pd.ent : -1;
+    int m = 1, rest = pd.next;
+    while (rest >= 0 && m < kSortedPending) {
+        if (lane == m) mine = rest;
+        ++m;
+        rest = __builtin_amdgcn_readfirstlane(d.pend_next[rest]);
+    }
+    int rank = 0;                                                  // position in ascending order
+    for (int a = 0; a < m; ++a) {
+        const int o = __shfl(mine, a);
+        if (lane < m && o < mine) ++rank;                          // entries are distinct
+    }
+    for (int a = 0;; ++a) {
+        Pending pa;
+        if (a < m) {
+            const int e = __shfl(mine, __builtin_ctzll(__ballot(lane < m && rank == a)));
+            pa = (e == pd.ent) ? pd : fetch_pending(d, e, from, lane);
+        } else if (rest >= 0) {                                    // beyond kSortedPending: queue order
+            pa = fetch_pending(d, rest, from, lane);
+            rest = pa.next;
+        } else {
+            break;
         }
-        wave_lds_sync();                                           // kk is rewritten next
-        if (nxt < 0) break;
+        subtract_ball(d, Dc, pa, u, kk, tab, mq, lane, offs);
     }
     double2* __restrict__ col = reinterpret_cast<double2*>(d.U) + (size_t)t * P;
 #pragma unroll
@@ -259,9 +376,11 @@ __global__ __launch_bounds__(kSlotThreads) void k_peel_flush(const qsft_peel_des
     __shared__ __align__(16) uint32_t kws[kSlotThreads / 64][QSFT_MAX_N / 4];
     const int inst = find_inst(pre, n_inst, blockIdx.x);
     const qsft_peel_desc& d = descs[inst];
+    __shared__ __align__(16) int8_t offbuf[kStageOffBytes];
     const uint32_t iters = d.ctr[QSFT_CTR_ITERS];
     if (iters == 0) return;
     fill_wtable_d(tab, d.q);
+    const int8_t* offs = stage_offsets(d, offbuf);
     __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int CQ = d.C * d.Q, P = d.P;
@@ -275,7 +394,7 @@ __global__ __launch_bounds__(kSlotThreads) void k_peel_flush(const qsft_peel_des
         double2 u[kRowsPerLane];
         load_col(u, reinterpret_cast<const double2*>(d.U) + (size_t)t * P, P, lane);
         const Pending pd = fetch_pending(d, d.pend_head[t], (int)iters - 1, lane);
-        apply_pending(d, t, t / d.Q, (int)iters - 1, pd, u, kws[wave], tab, mq, lane);
+        apply_pending(d, t, t / d.Q, (int)iters - 1, pd, u, kws[wave], tab, mq, lane, offs);
     }
 }
 
@@ -294,6 +413,8 @@ __global__ __launch_bounds__(kBinThreads) __attribute__((amdgpu_waves_per_eu(3))
     __shared__ uint64_t sb_h[kWaves][kBinsPerWave];
     __shared__ double2 sb_rho[kWaves][kBinsPerWave];
     __shared__ int32_t sb_t[kWaves][kBinsPerWave];
+    __shared__ __align__(16) int8_t offbuf[kStageOffBytes];
+    __shared__ __align__(16) int8_t mtbuf[kStageMtBytes];
     const int inst = find_inst(pre, n_inst, blockIdx.x);
     const qsft_peel_desc& d = descs[inst];
     uint32_t* __restrict__ ctr = d.ctr;
@@ -302,6 +423,8 @@ __global__ __launch_bounds__(kBinThreads) __attribute__((amdgpu_waves_per_eu(3))
     const int q = d.q, n = d.n, b = d.b, C = d.C, P = d.P, Q = d.Q, n_pad = d.n_pad;
     const int nw = n_pad >> 2;
     fill_wtable_d(tab, q);
+    const int8_t* offs = stage_offsets(d, offbuf);
+    const int8_t* mtab = stage_mt(d, mtbuf);
     __syncthreads();
 
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -366,7 +489,7 @@ __global__ __launch_bounds__(kBinThreads) __attribute__((amdgpu_waves_per_eu(3))
 
         const int c = t / Q, j = t - c * Q;
         // balls queued on this bin by the previous pass
-        apply_pending(d, t, c, it - 1, pd, u, kk, tab, mq, lane);
+        apply_pending(d, t, c, it - 1, pd, u, kk, tab, mq, lane, offs);
         if (bin_energy(u) > thr) {                                 // not a zeroton (wave-uniform)
             ++nz;
             if (pd.ent >= 0) {                                     // the column changed: refresh the LDS copy
@@ -430,13 +553,12 @@ __global__ __launch_bounds__(kBinThreads) __attribute__((amdgpu_waves_per_eu(3))
             }
             const int8_t* __restrict__ Dc = d.D + (size_t)c * P * n_pad;
             double2 w[kRowsPerLane];
+            row_phases(d, Dc, kk, tab, mq, lane, w, offs);
             double rr = 0.0, ri = 0.0;
 #pragma unroll
             for (int r = 0; r < kRowsPerLane; ++r) {
                 const int p = lane + 64 * r;
-                w[r] = make_double2(0.0, 0.0);
                 if (p < P) {
-                    w[r] = tab[mq(dot_row(Dc + (size_t)p * n_pad, kk, nw))];
                     rr += w[r].x * u[r].x + w[r].y * u[r].y;       // conj(w) * u
                     ri += w[r].x * u[r].y - w[r].y * u[r].x;
                 }
@@ -465,7 +587,8 @@ __global__ __launch_bounds__(kBinThreads) __attribute__((amdgpu_waves_per_eu(3))
             bool ok = true;
             if (lane < b) {
                 const int dig = (j / digit_weight(q, b, lane)) % q;
-                ok = (int)mq(dot_row(d.Mt + ((size_t)c * b + lane) * n_pad, kk, nw)) == dig;
+                const int8_t* mrow = mtab + ((size_t)c * b + lane) * n_pad;
+                ok = (int)mq(mtab == d.Mt ? dot_row(mrow, kk, nw) : dot_row_lds(mrow, kk, nw)) == dig;
             }
             const bool match = __all(ok);
             const bool single = match && res <= thr;

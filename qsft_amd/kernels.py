@@ -346,13 +346,33 @@ def assemble(items):
 # -------------------------------------------------------------------- stage (d)
 
 
+def unit_delay_sign(D, q, n, p1):
+    """+1 or -1 if in every group each repeat block of delay rows is its row 0
+    plus s e_(i-1) (mod q) in row i (the identity and NSO delays of the
+    reference's get_D), with zero padding past n; 0 otherwise (the peeler then
+    takes a full dot product per row). D: int8 [C][P][n_pad] on the device."""
+    C, P, n_pad = D.shape
+    if p1 != n + 1 or P % p1 or P // p1 > 64 or bool(D[:, :, n:].any()):
+        return 0
+    R = P // p1
+    Dn = D[:, :, :n].to(torch.int64).reshape(C, R, p1, n)
+    E = torch.zeros(p1, n, dtype=torch.int64, device=D.device)
+    E[1:] = torch.eye(n, dtype=torch.int64, device=D.device)
+    for sgn in (-1, 1):
+        if torch.equal((Dn[:, :, :1, :] + sgn * E) % q, Dn):
+            return sgn
+    return 0
+
+
 class PeelBatch:
     """Bound peel descriptors + workspaces for a list of instances.
 
     items: dicts q, n, b, C, P, p1, method, cutoff, Mt, D, U (complex128,
     bin-major [C][Q][P]), optional shared_d (all groups use the delay rows of
-    group 0; compared on the device when absent) and energy (float64 [C][Q]
-    first-pass column energies written by Assemble into the same U).
+    group 0; compared on the device when absent), unit_d (+1 / -1: delay rows
+    are block offsets + s e_i, 0: arbitrary rows; detected when absent) and
+    energy (float64 [C][Q] first-pass column energies written by Assemble into
+    the same U).
     """
 
     def __init__(self, items, max_iter=15, host_rows=False, cap_rows=None):
@@ -382,6 +402,11 @@ class PeelBatch:
                 Dg = it["D"]
                 shared = all(torch.equal(Dg[0], Dg[c]) for c in range(1, C))
             d.flags = L.QSFT_PEEL_SHARED_D if shared else 0
+            unit = it.get("unit_d")
+            if unit is None:
+                unit = unit_delay_sign(it["D"][:1] if shared else it["D"], q, n, it["p1"])
+            if unit:
+                d.flags |= L.QSFT_PEEL_UNIT_D | (L.QSFT_PEEL_UNIT_D_NEG if unit < 0 else 0)
             if it.get("energy") is not None:
                 d.energy = _ptr(it["energy"])
             self.descs.append(d)

@@ -23,8 +23,9 @@ if torch.cuda.is_available():
     from qsft_amd.synthetic_signal import get_random_subsampled_signal
 
 
-def device_peel(Ms, Ds, Us, q, n, cutoff, method="nso", host_loop=False):
-    """Run the device peeler on numpy inputs; returns (gwht dict, info, final U)."""
+def device_peel(Ms, Ds, Us, q, n, cutoff, method="nso", host_loop=False, unit_d=None):
+    """Run the device peeler on numpy inputs; returns (gwht dict, info, final U).
+    unit_d=0 forces the full dot product per delay row."""
     dev = dv.current_device()
     C, P, Q = np.asarray(Us).shape
     b = Ms[0].shape[1]
@@ -34,6 +35,8 @@ def device_peel(Ms, Ds, Us, q, n, cutoff, method="nso", host_loop=False):
     item = dict(q=q, n=n, b=b, C=C, P=P, p1=n + 1,
                 method=L.QSFT_DETECT_NSO1 if method == "nso" else L.QSFT_DETECT_NOISELESS,
                 cutoff=cutoff, Mt=dv.Mt_tensor(Ms, b, n_pad, dev), D=dv.D_tensor(Ds, n_pad, dev), U=U)
+    if unit_d is not None:
+        item["unit_d"] = unit_d
     batch = K.PeelBatch([item])
     batch.run(host_loop=host_loop)
     k, v, _, info = batch.results()[0]
@@ -295,3 +298,32 @@ def test_k1_noisy_fp32_nmse():
     nm = O.nmse(gw, sig.signal_w)
     print(f"K1 10 dB NMSE {nm:.3e}, recovered {len(gw)}", flush=True)
     assert nm <= 1e-2
+
+
+@pytest.mark.parametrize("channel", ["nso", "identity", "random"])
+def test_unit_delay_signatures_match_row_products(channel):
+    """The structured-delay signature (R offset products + one digit per row)
+    gives the same decode as a full dot product per row; a random delay matrix
+    is detected as unstructured."""
+    q, n, b, C, S = 4, 12, 4, 3, 60
+    rng = np.random.default_rng(31)
+    Ms = [rng.integers(0, q, size=(n, b)) for _ in range(C)]
+    if channel == "nso":
+        D, R = _nso_D(q, n, 2, rng), 2
+    elif channel == "identity":
+        D, R = np.vstack([np.zeros(n), np.eye(n)]).astype(int), 1
+    else:
+        D, R = rng.integers(0, q, size=(2 * (n + 1), n)), 2
+    locq = rng.integers(0, q, size=(n, S))
+    a = rng.uniform(1, 10, S) * np.exp(2j * np.pi * rng.uniform(size=S))
+    Us = np.array([O.closed_form_U(M, D, locq, a, q) for M in Ms])
+    method = "identity" if channel == "identity" else "nso"
+    print(f"unit delays {channel}", flush=True)
+    n_pad = dv.n_pad_of(n)
+    sign = K.unit_delay_sign(dv.D_tensor([D] * C, n_pad, dv.current_device()), q, n, n + 1)
+    assert sign == {"nso": -1, "identity": 1, "random": 0}[channel]
+    got, info, U1 = device_peel(Ms, [D] * C, Us.copy(), q, n, 1e-9, method=method)
+    ref, info0, U0 = device_peel(Ms, [D] * C, Us.copy(), q, n, 1e-9, method=method, unit_d=0)
+    assert_same_result(got, ref, rtol=0.0)
+    assert info["trace"] == info0["trace"]
+    np.testing.assert_array_equal(U1, U0)
