@@ -167,6 +167,13 @@ int qsft_dft_lds_bytes(const qsft_dft_desc* host_desc, int pass, int precision);
  * instance's pass; smaller tiles leave more workgroups resident per CU. */
 int qsft_dft_pass(const qsft_dft_desc* dev_descs, const int32_t* dev_pre, int n_inst,
                   int total_blocks, int pass, int precision, int lds_bytes, void* stream);
+/* q = 4 and b in {4, 6} with in_stride 1 (the served sub-transforms): one
+ * workgroup per row (dev_pre counts rows), the axis transforms two digits at a
+ * time in registers, one launch instead of qsft_dft_pass's passes. word: low
+ * nibble QSFT_F32 / QSFT_F64, bits 8..15 = b (all instances of the launch).
+ * Same reference code as qsft_dft_pass. */
+int qsft_dft4_rows(const qsft_dft_desc* dev_descs, const int32_t* dev_pre, int n_inst, int total_blocks,
+                   int word, void* stream);
 
 /* ------------------------------------------------------------------------
  * Working transform for one QSFT.transform call: gather the selected groups

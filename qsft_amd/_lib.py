@@ -96,6 +96,7 @@ SIGNATURES = {
     "qsft_dft_lds_bytes": (ctypes.c_int, [ctypes.POINTER(DftDesc), ctypes.c_int, ctypes.c_int]),
     "qsft_dft_pass": (ctypes.c_int, [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                      vp]),
+    "qsft_dft4_rows": (ctypes.c_int, [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp]),
     "qsft_assemble_blocks": (ctypes.c_int, [ctypes.POINTER(AssembleDesc)]),
     "qsft_assemble": (ctypes.c_int, [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_uint64, vp]),
     "qsft_peel_workspace_bytes": (ctypes.c_size_t, [ctypes.c_int] * 7),

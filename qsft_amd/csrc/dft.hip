@@ -189,6 +189,80 @@ __global__ __launch_bounds__(kDftThreads) void k_dft_pass(const qsft_dft_desc* _
     }
 }
 
+// q = 4, whole rows of Q = 4^B samples (B even), one workgroup per row,
+// 4^B / 16 threads. Base-4 digit t of the element index sits at bits 2t, 2t+1.
+// A thread holds 16 samples that differ in two digits, so two of the axis
+// transforms run in registers; two digits per register pass, B / 2 passes,
+// the samples re-dealt through LDS between passes (one pad element per 16
+// keeps the strided LDS accesses spread over the banks). The first pass reads
+// 16 consecutive samples per thread, the last stores 64 consecutive samples
+// per wave instruction. Same 4-point butterflies and scaling as k_dft_pass.
+template <typename T>
+__device__ __forceinline__ void dft4(cplx_t<T>& x0, cplx_t<T>& x1, cplx_t<T>& x2, cplx_t<T>& x3) {
+    using C2 = cplx_t<T>;
+    const C2 s02{x0.x + x2.x, x0.y + x2.y}, d02{x0.x - x2.x, x0.y - x2.y};
+    const C2 s13{x1.x + x3.x, x1.y + x3.y}, d13{x1.x - x3.x, x1.y - x3.y};
+    x0 = C2{s02.x + s13.x, s02.y + s13.y};
+    x2 = C2{s02.x - s13.x, s02.y - s13.y};
+    x1 = C2{d02.x + d13.y, d02.y - d13.x};                        // d02 + (-i) d13
+    x3 = C2{d02.x - d13.y, d02.y + d13.x};
+}
+
+// element index of register j of thread tid when the thread holds digits 2p, 2p+1
+__device__ __forceinline__ int dft4_index(int p, int tid, int j) {
+    const int lo = 4 * p;
+    return ((tid >> lo) << (lo + 4)) | (j << lo) | (tid & ((1 << lo) - 1));
+}
+__device__ __forceinline__ int dft4_pad(int i) { return i + (i >> 4); }
+
+template <typename T, int B>
+__global__ __launch_bounds__(1 << (2 * B - 4)) void k_dft4_rows(const qsft_dft_desc* __restrict__ descs,
+                                                                const int32_t* __restrict__ pre, int n_inst) {
+    using C2 = cplx_t<T>;
+    constexpr int Q = 1 << (2 * B);
+    static_assert(B % 2 == 0 && B >= 4, "even digit count");
+    __shared__ C2 lds[Q + Q / 16];
+    const int inst = find_inst(pre, n_inst, blockIdx.x);
+    const qsft_dft_desc& d = descs[inst];
+    const int row = blockIdx.x - pre[inst];
+    if (d.q != 4 || d.b != B || d.in_stride != 1 || row >= d.rows) return;   // launched for other shapes
+    const int tid = threadIdx.x;
+    const C2* __restrict__ in = reinterpret_cast<const C2*>(d.in) + (int64_t)row * d.in_row_stride;
+    C2* __restrict__ out = reinterpret_cast<C2*>(d.out) + (int64_t)row * Q;
+
+    C2 v[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) v[j] = in[tid * 16 + j];
+#pragma unroll
+    for (int p = 0; p < B / 2; ++p) {
+        if (p > 0) {
+            __syncthreads();                                       // the previous reads of lds are done
+#pragma unroll
+            for (int j = 0; j < 16; ++j) lds[dft4_pad(dft4_index(p - 1, tid, j))] = v[j];
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < 16; ++j) v[j] = lds[dft4_pad(dft4_index(p, tid, j))];
+        }
+#pragma unroll
+        for (int a = 0; a < 4; ++a) dft4<T>(v[4 * a], v[4 * a + 1], v[4 * a + 2], v[4 * a + 3]);   // digit 2p
+#pragma unroll
+        for (int a = 0; a < 4; ++a) dft4<T>(v[a], v[a + 4], v[a + 8], v[a + 12]);                 // digit 2p+1
+    }
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        C2 x = v[j];
+        if constexpr (sizeof(T) == 8) {
+            x.x = x.x / (double)Q;
+            x.y = x.y / (double)Q;
+        } else {
+            const T sc = (T)(1.0 / (double)Q);
+            x.x *= sc;
+            x.y *= sc;
+        }
+        out[dft4_index(B / 2 - 1, tid, j)] = x;
+    }
+}
+
 }  // namespace qsft
 
 using namespace qsft;
@@ -284,4 +358,26 @@ extern "C" int qsft_dft_pass(const qsft_dft_desc* d, const int32_t* pre, int n_i
     const hipError_t e = (prec == QSFT_F32) ? launch_dft<float>(radix, d, pre, n_inst, blocks, pass, lds_bytes, st)
                                             : launch_dft<double>(radix, d, pre, n_inst, blocks, pass, lds_bytes, st);
     return qsft_hip_status(e, "qsft_dft_pass");
+}
+
+// q = 4 rows of 4^b samples, b in {4, 6}, contiguous input (in_stride 1):
+// one workgroup per row. word: low nibble QSFT_F32/F64, bits 8..15 = b.
+extern "C" int qsft_dft4_rows(const qsft_dft_desc* d, const int32_t* pre, int n_inst, int blocks, int word,
+                              void* stream) {
+    const int prec = word & 0xF, b = (word >> 8) & 0xFF;
+    if (!d || !pre || n_inst < 1 || blocks < 0 || (prec != QSFT_F32 && prec != QSFT_F64) || (b != 4 && b != 6)) {
+        qsft_set_error("qsft_dft4_rows: bad arguments");
+        return QSFT_ERR_ARG;
+    }
+    if (blocks == 0) return QSFT_OK;
+    hipStream_t st = (hipStream_t)stream;
+    const dim3 g(blocks);
+    if (prec == QSFT_F32) {
+        if (b == 6) hipLaunchKernelGGL((k_dft4_rows<float, 6>), g, dim3(256), 0, st, d, pre, n_inst);
+        else hipLaunchKernelGGL((k_dft4_rows<float, 4>), g, dim3(16), 0, st, d, pre, n_inst);
+    } else {
+        if (b == 6) hipLaunchKernelGGL((k_dft4_rows<double, 6>), g, dim3(256), 0, st, d, pre, n_inst);
+        else hipLaunchKernelGGL((k_dft4_rows<double, 4>), g, dim3(16), 0, st, d, pre, n_inst);
+    }
+    return qsft_hip_status(hipGetLastError(), "qsft_dft4_rows");
 }

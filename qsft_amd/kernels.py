@@ -253,13 +253,25 @@ def eval_points(items, precision):
 
 
 class Dft:
-    """items: dicts q, b, rows, inp (tensor), in_stride, in_row_stride, out (tensor [rows][q^b])."""
+    """items: dicts q, b, rows, inp (tensor), in_stride, in_row_stride, out (tensor [rows][q^b]).
+    rows4: q = 4, b in {4, 6} contiguous rows go to the whole-row kernel."""
 
-    def __init__(self, items, precision):
+    def __init__(self, items, precision, rows4=True):
         groups = defaultdict(list)
+        fast = defaultdict(list)
         for it in items:
-            groups[it["q"]].append(it)
+            if rows4 and it["q"] == 4 and it["b"] in (4, 6) and it.get("in_stride", 1) == 1:
+                fast[it["b"]].append(it)                           # whole rows in one launch
+            else:
+                groups[it["q"]].append(it)
         self.launches = []
+        self.rows4 = []
+        for b, its in fast.items():
+            descs = [L.DftDesc(q=4, b=b, rows=it["rows"], in_stride=1,
+                               in_row_stride=it.get("in_row_stride", 4 ** b), in_=_ptr(it["inp"]),
+                               out=_ptr(it["out"])) for it in its]
+            buf, dptr, (pre,), (tot,) = pack(descs, [[it["rows"] for it in its]], its[0]["out"].device)
+            self.rows4.append((buf, dptr, pre, tot, len(descs), precision | (b << 8)))
         for q, its in groups.items():
             descs = []
             for it in its:
@@ -278,6 +290,8 @@ class Dft:
             self.launches.append((buf, dptr, pres, tots, len(descs), precision | (q << 8), lds))
 
     def launch(self):
+        for _, dptr, pre, tot, n, word in self.rows4:
+            L.check(lib.qsft_dft4_rows(_vp(dptr), _vp(pre), n, tot, word, stream_handle()), "qsft_dft4_rows")
         for _, dptr, pres, tots, n, word, lds in self.launches:
             for p in range(len(pres)):
                 L.check(lib.qsft_dft_pass(_vp(dptr), _vp(pres[p]), n, tots[p], p, word, lds[p], stream_handle()),

@@ -178,6 +178,30 @@ def test_dft_matches_fftn(q, b, rows, precision):
     assert rel_err(got, ref) < (1e-13 if precision == "fp64" else 3e-6)
 
 
+@pytest.mark.parametrize("b", [4, 6])
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_dft_whole_row_kernel_matches_pass_kernel(b, precision):
+    """q = 4 rows through the whole-row kernel and through the per-pass kernel
+    agree with each other and with the DFT definition."""
+    print(f"dft rows4 b={b} {precision}", flush=True)
+    rng = np.random.default_rng(13)
+    rows, q = 7, 4
+    x = rng.normal(size=(rows, q ** b)) + 1j * rng.normal(size=(rows, q ** b))
+    dt = torch.complex128 if precision == "fp64" else torch.complex64
+    code = dv.PRECISIONS[precision][0]
+    dev = dv.current_device()
+    outs = []
+    for rows4 in (True, False):
+        xt = torch.from_numpy(x).to(dt).to(dev)
+        yt = torch.empty_like(xt)
+        K.Dft([dict(q=q, b=b, rows=rows, inp=xt, out=yt)], code, rows4=rows4).launch()
+        outs.append(yt.cpu().numpy())
+    ref = np.stack([O.gwht(r, q, b) for r in x])
+    tol = 1e-13 if precision == "fp64" else 3e-6
+    assert rel_err(outs[0], ref) < tol and rel_err(outs[1], ref) < tol
+    assert rel_err(outs[0], outs[1]) < tol
+
+
 @pytest.mark.parametrize("q,b,bb", [(4, 6, 4), (3, 5, 2), (20, 3, 2)])
 def test_dft_strided_sub_b(q, b, bb):
     """all_bs: the b' < b transform of row[::q^(b-b')]."""
