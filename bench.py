@@ -193,9 +193,11 @@ def main():
         flops = 8.0 * samples * S * args.batch
         achieved = flops / eval_s / 1e12
         # executed on the matrix cores: 2 fp16 limbs of the (Q x 2S) x (2S x 2P) real GEMM
-        # (padded to 16-support blocks and 8-row tiles, 8 tiles per workgroup) plus the int8 E0 products
+        # (padded to 16-support blocks and 8-row tiles, ptb tiles per workgroup) plus the int8 E0 products
+        from qsft_amd.kernels import mfma_mode, mfma_ptb
         nsb, npt = (S + 15) // 16, (R * (n + 1) + 7) // 8
-        npt_run = 8 * ((npt + 7) // 8)
+        ptb = mfma_ptb(mfma_mode(q), 1, npt)
+        npt_run = ptb * ((npt + ptb - 1) // ptb)
         mfma_f16 = 2 * 2.0 * C * q ** b * (32 * nsb) * (16 * npt_run) * args.batch
         roof = {"kernel": "k_eval_mfma (int8 MFMA phase + fp16x2 MFMA complex GEMM, FP32 accumulation)",
                 "bound": "mfma", "achieved": achieved, "peak": PEAK_F16_TFLOPS, "unit": "TFLOP/s",

@@ -98,7 +98,8 @@ typedef struct qsft_eval_mfma_desc {
     int32_t shared_d;       /* 1: every group has the same delay rows (B computed once)     */
     int32_t bexp;           /* 14 - e where max|a_s| = m 2^e, m in [0.5, 1): the largest    *
                              * scaled |B| lies in [2^13, 2^14); in [-100, 100]               */
-    int32_t _pad1;
+    int32_t ptb;            /* p-tiles per workgroup: 5..8 for q in {2, 4} with n_pad = 64,  *
+                             * 8 otherwise (same for every instance of a launch)           */
     const int8_t* Mt;       /* [C][b][n_pad]                                                */
     const int8_t* D;        /* [C][P][n_pad]                                                */
     const int8_t* K;        /* [S][n_pad]                                                   */

@@ -71,7 +71,7 @@ class PeelDesc(ctypes.Structure):
 
 class EvalMfmaDesc(ctypes.Structure):
     _fields_ = [("q", i32), ("n", i32), ("b", i32), ("C", i32), ("P", i32), ("S", i32),
-                ("n_pad", i32), ("nsb", i32), ("npt", i32), ("shared_d", i32), ("bexp", i32), ("_pad1", i32),
+                ("n_pad", i32), ("nsb", i32), ("npt", i32), ("shared_d", i32), ("bexp", i32), ("ptb", i32),
                 ("Mt", vp), ("D", vp), ("K", vp), ("a", vp), ("Bf", vp), ("x", vp)]
 
 

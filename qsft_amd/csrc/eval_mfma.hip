@@ -46,7 +46,7 @@ typedef _Float16 v8h __attribute__((ext_vector_type(8)));
 
 constexpr int kMfThreads = 256;
 constexpr int kRowsPerBlock = 256;   // base queries per workgroup
-constexpr int kPtPerBlock = 8;       // p-tiles (8 complex delay rows each) per workgroup
+constexpr int kPtPerBlock = 8;       // most p-tiles (8 complex delay rows each) per workgroup
 constexpr int kLimbs = 2;            // fp16 limbs of B (and of W for q not in {2, 4})
 constexpr int kStage = kPtPerBlock * kLimbs * 64;      // v4u per staged support block
 constexpr int kStagePerThread = kStage / kMfThreads;   // 4
@@ -179,7 +179,7 @@ __device__ __forceinline__ void ring_barrier() {
 // registers hold staged data. The slots are separate arrays and the loop is
 // unrolled by four, so every LDS access names its slot. One wait and one
 // barrier per pair of support blocks.
-template <int MODE, int NKB>
+template <int MODE, int NKB, int NPT>
 __device__ __forceinline__ void eval_mfma_body(const qsft_eval_mfma_desc* __restrict__ descs,
                                                const int32_t* __restrict__ pre, int n_inst) {
     constexpr int WL = (MODE == 0) ? 1 : kLimbs;
@@ -190,18 +190,19 @@ __device__ __forceinline__ void eval_mfma_body(const qsft_eval_mfma_desc* __rest
     const int lblock = xcd_logical_block();
     const int inst = find_inst(pre, n_inst, lblock);
     const qsft_eval_mfma_desc& d = descs[inst];
+    if (d.ptb != NPT) return;                                      // launched for another split (host error)
     const int q = d.q, b = d.b, n_pad = d.n_pad, nsb = d.nsb, npt = d.npt, P = d.P, S = d.S;
     const int Q = ipow(q, b);
     const int nlb = (Q + kRowsPerBlock - 1) / kRowsPerBlock;
-    const int ncg = (npt + kPtPerBlock - 1) / kPtPerBlock;
+    const int ncg = (npt + NPT - 1) / NPT;
     int local = lblock - pre[inst];
     const int cg = local % ncg;
     local /= ncg;
     const int lb = local % nlb;
     const int c = local / nlb;
     const int cb = d.shared_d ? 0 : c;                            // group whose B fragments apply
-    const int pt0 = cg * kPtPerBlock;
-    const int npt_here = min(kPtPerBlock, npt - pt0);
+    const int pt0 = cg * NPT;
+    const int npt_here = min(NPT, npt - pt0);
     const int nstage = npt_here * kLimbs * 64;
 
     for (int r = threadIdx.x; r < q; r += blockDim.x) {
@@ -246,11 +247,11 @@ __device__ __forceinline__ void eval_mfma_body(const qsft_eval_mfma_desc* __rest
         }
     }
 
-    v4f acc[4][kPtPerBlock];
+    v4f acc[4][NPT];
 #pragma unroll
     for (int t = 0; t < 4; ++t)
 #pragma unroll
-        for (int pt = 0; pt < kPtPerBlock; ++pt) acc[t][pt] = v4f{0.f, 0.f, 0.f, 0.f};
+        for (int pt = 0; pt < NPT; ++pt) acc[t][pt] = v4f{0.f, 0.f, 0.f, 0.f};
 
     const v4u* __restrict__ Bg = reinterpret_cast<const v4u*>(d.Bf);
     const int8_t* __restrict__ Kd = d.K;
@@ -340,9 +341,9 @@ __device__ __forceinline__ void eval_mfma_body(const qsft_eval_mfma_desc* __rest
                 // flight while the MFMAs of p-tile pt run
                 v4u bnext = bcur[bl * 64 + lane];
 #pragma unroll
-                for (int pt = 0; pt < kPtPerBlock; ++pt) {
+                for (int pt = 0; pt < NPT; ++pt) {
                     const v8h bf = __builtin_bit_cast(v8h, bnext);
-                    if (pt + 1 < kPtPerBlock) bnext = bcur[((pt + 1) * kLimbs + bl) * 64 + lane];
+                    if (pt + 1 < NPT) bnext = bcur[((pt + 1) * kLimbs + bl) * 64 + lane];
 #pragma unroll
                     for (int t = 0; t < 4; ++t)
                         acc[t][pt] = __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(v8h, wa[t][wl]), bf,
@@ -389,7 +390,7 @@ __device__ __forceinline__ void eval_mfma_body(const qsft_eval_mfma_desc* __rest
     for (int t = 0; t < 4; ++t) {
         const int l0 = lbase + 16 * t + 4 * g;
 #pragma unroll
-        for (int pt = 0; pt < kPtPerBlock; ++pt) {
+        for (int pt = 0; pt < NPT; ++pt) {
             if (pt < npt_here) {
                 const v4f a = acc[t][pt];
                 v4f o;
@@ -416,16 +417,18 @@ __device__ __forceinline__ void eval_mfma_body(const qsft_eval_mfma_desc* __rest
 template <int MODE, int NKB>
 __global__ __launch_bounds__(kMfThreads) void k_eval_mfma(const qsft_eval_mfma_desc* __restrict__ descs,
                                                           const int32_t* __restrict__ pre, int n_inst) {
-    eval_mfma_body<MODE, NKB>(descs, pre, n_inst);
+    eval_mfma_body<MODE, NKB, kPtPerBlock>(descs, pre, n_inst);
 }
 
 // Two waves per SIMD: caps the body at 256 VGPR+AGPR. Only the one-limb phase
 // (q in {2, 4}) with one k-block (n <= 64) fits that without scratch; it then
 // hides the LDS and global latency of one block behind the MFMAs of the other.
-template <int MODE, int NKB>
+// NPT (5..8) p-tiles per workgroup, chosen so that the tiles of the delay rows
+// split evenly over the workgroups (13 tiles: 2 x 7, not 2 x 8).
+template <int MODE, int NKB, int NPT>
 __global__ __launch_bounds__(kMfThreads) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_eval_mfma_occ2(
     const qsft_eval_mfma_desc* __restrict__ descs, const int32_t* __restrict__ pre, int n_inst) {
-    eval_mfma_body<MODE, NKB>(descs, pre, n_inst);
+    eval_mfma_body<MODE, NKB, NPT>(descs, pre, n_inst);
 }
 
 }  // namespace qsft
@@ -439,7 +442,7 @@ static bool mfma_desc_ok(const qsft_eval_mfma_desc* h) {
     return h && h->q >= 2 && h->q <= QSFT_MAX_Q && h->b >= 1 && h->b <= QSFT_MAX_B && h->n >= 1 && h->n <= h->n_pad &&
            h->n_pad % 64 == 0 && h->n_pad <= QSFT_MAX_N && h->C >= 1 && h->P >= 1 && h->S >= 1 &&
            h->nsb == (h->S + 15) / 16 && h->npt == (h->P + 7) / 8 && (h->shared_d == 0 || h->shared_d == 1) &&
-           h->bexp >= -100 && h->bexp <= 100;
+           h->bexp >= -100 && h->bexp <= 100 && h->ptb >= 1 && h->ptb <= kPtPerBlock;
 }
 
 extern "C" size_t qsft_eval_mfma_workspace_bytes(int C, int P, int S) {
@@ -453,7 +456,7 @@ extern "C" int qsft_eval_mfma_blocks(const qsft_eval_mfma_desc* h, int* prep_blo
     const long long Cb = h->shared_d ? 1 : h->C;
     const long long prep = (Cb * h->nsb * h->npt * 32 + 255) / 256;
     const long long mainb = (long long)h->C * ((Q + kRowsPerBlock - 1) / kRowsPerBlock) *
-                            ((h->npt + kPtPerBlock - 1) / kPtPerBlock);
+                            ((h->npt + h->ptb - 1) / h->ptb);
     if (prep > 0x7FFFFFFF || mainb > 0x7FFFFFFF) return -1;
     *prep_blocks = (int)prep;
     *main_blocks = (int)mainb;
@@ -461,14 +464,20 @@ extern "C" int qsft_eval_mfma_blocks(const qsft_eval_mfma_desc* h, int* prep_blo
 }
 
 template <int MODE>
-static hipError_t launch_mfma(int nkb, const qsft_eval_mfma_desc* d, const int32_t* pre, int n, int blocks,
+static hipError_t launch_mfma(int nkb, int ptb, const qsft_eval_mfma_desc* d, const int32_t* pre, int n, int blocks,
                               hipStream_t st) {
     switch (nkb) {
         case 1:
-            if constexpr (MODE == 0)
-                hipLaunchKernelGGL((k_eval_mfma_occ2<MODE, 1>), dim3(blocks), dim3(kMfThreads), 0, st, d, pre, n);
-            else
+            if constexpr (MODE == 0) {
+                switch (ptb) {
+                    case 5: hipLaunchKernelGGL((k_eval_mfma_occ2<0, 1, 5>), dim3(blocks), dim3(kMfThreads), 0, st, d, pre, n); break;
+                    case 6: hipLaunchKernelGGL((k_eval_mfma_occ2<0, 1, 6>), dim3(blocks), dim3(kMfThreads), 0, st, d, pre, n); break;
+                    case 7: hipLaunchKernelGGL((k_eval_mfma_occ2<0, 1, 7>), dim3(blocks), dim3(kMfThreads), 0, st, d, pre, n); break;
+                    default: hipLaunchKernelGGL((k_eval_mfma_occ2<0, 1, 8>), dim3(blocks), dim3(kMfThreads), 0, st, d, pre, n); break;
+                }
+            } else {
                 hipLaunchKernelGGL((k_eval_mfma<MODE, 1>), dim3(blocks), dim3(kMfThreads), 0, st, d, pre, n);
+            }
             break;
         case 2: hipLaunchKernelGGL((k_eval_mfma<MODE, 2>), dim3(blocks), dim3(kMfThreads), 0, st, d, pre, n); break;
         case 3: hipLaunchKernelGGL((k_eval_mfma<MODE, 3>), dim3(blocks), dim3(kMfThreads), 0, st, d, pre, n); break;
@@ -477,7 +486,8 @@ static hipError_t launch_mfma(int nkb, const qsft_eval_mfma_desc* d, const int32
     return hipGetLastError();
 }
 
-// word: bits 4..7 digit-block count (1..4), bits 8..9 mode (0: q in {2,4}; 1: other powers of two; 2: other q).
+// word: bits 4..7 digit-block count (1..4), bits 8..9 mode (0: q in {2,4}; 1: other powers of two; 2: other q),
+// bits 12..15 p-tiles per workgroup (the descriptors' ptb; 5..8 for mode 0 with one digit block, else 8).
 // All instances of one launch share the word.
 extern "C" int qsft_eval_mfma(const qsft_eval_mfma_desc* d, const int32_t* pre_prep, const int32_t* pre_main,
                               int n_inst, int prep_blocks, int main_blocks, int word, void* stream) {
@@ -485,8 +495,12 @@ extern "C" int qsft_eval_mfma(const qsft_eval_mfma_desc* d, const int32_t* pre_p
         qsft_set_error("qsft_eval_mfma: bad arguments");
         return QSFT_ERR_ARG;
     }
-    const int nkb = (word >> 4) & 0xF, mode = (word >> 8) & 0x3;
-    if (nkb < 1 || nkb > 4 || mode > 2) { qsft_set_error("qsft_eval_mfma: bad word"); return QSFT_ERR_ARG; }
+    const int nkb = (word >> 4) & 0xF, mode = (word >> 8) & 0x3, ptb = (word >> 12) & 0xF;
+    const bool split_ok = (mode == 0 && nkb == 1) ? (ptb >= 5 && ptb <= kPtPerBlock) : (ptb == kPtPerBlock);
+    if (nkb < 1 || nkb > 4 || mode > 2 || !split_ok) {
+        qsft_set_error("qsft_eval_mfma: bad word");
+        return QSFT_ERR_ARG;
+    }
     hipStream_t st = (hipStream_t)stream;
     if (prep_blocks > 0) {
         hipLaunchKernelGGL(k_eval_mfma_prep, dim3(prep_blocks), dim3(256), 0, st, d, pre_prep, n_inst);
@@ -495,8 +509,8 @@ extern "C" int qsft_eval_mfma(const qsft_eval_mfma_desc* d, const int32_t* pre_p
     }
     if (main_blocks == 0) return QSFT_OK;
     hipError_t e;
-    if (mode == 0) e = launch_mfma<0>(nkb, d, pre_main, n_inst, main_blocks, st);
-    else if (mode == 1) e = launch_mfma<1>(nkb, d, pre_main, n_inst, main_blocks, st);
-    else e = launch_mfma<2>(nkb, d, pre_main, n_inst, main_blocks, st);
+    if (mode == 0) e = launch_mfma<0>(nkb, ptb, d, pre_main, n_inst, main_blocks, st);
+    else if (mode == 1) e = launch_mfma<1>(nkb, ptb, d, pre_main, n_inst, main_blocks, st);
+    else e = launch_mfma<2>(nkb, ptb, d, pre_main, n_inst, main_blocks, st);
     return qsft_hip_status(e, "qsft_eval_mfma");
 }

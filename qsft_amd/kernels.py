@@ -116,6 +116,15 @@ def mfma_mode(q):
     return 1 if is_pow2(q) else 2
 
 
+def mfma_ptb(mode, nkb, npt):
+    """p-tiles per workgroup of the matrix-core evaluator: the tiles split as
+    evenly as possible over ceil(npt / 8) workgroups (5..8) for q in {2, 4}
+    with n <= 64, else 8."""
+    if mode != 0 or nkb != 1:
+        return 8
+    return max(5, -(-npt // -(-npt // 8)))
+
+
 def mfma_bexp(amax, a64):
     """Scale exponent of the fp16 limbs: 14 - e with max|a| = m 2^e, m in [0.5, 1)."""
     if amax is None:
@@ -138,10 +147,11 @@ class EvalMfma:
         self.empty = [it["x"] for it in items if it["S"] == 0]     # no support: x = 0
         for it in items:
             if it["S"] > 0:
-                groups[(it["Mt"].shape[-1] // 64, mfma_mode(it["q"]))].append(it)
+                nkb, mode = it["Mt"].shape[-1] // 64, mfma_mode(it["q"])
+                groups[(nkb, mode, mfma_ptb(mode, nkb, (it["P"] + 7) // 8))].append(it)
         self.launches = []
         self.ws = []
-        for (nkb, mode), its in groups.items():
+        for (nkb, mode, ptb), its in groups.items():
             descs, prep, main = [], [], []
             for it in its:
                 if it["x"].dtype != torch.complex64 or it["a64"].dtype != torch.complex128:
@@ -156,7 +166,7 @@ class EvalMfma:
                 self.ws.append(ws)
                 d = L.EvalMfmaDesc(q=it["q"], n=it["n"], b=it["b"], C=C, P=P, S=S, n_pad=it["Mt"].shape[-1],
                                    nsb=(S + 15) // 16, npt=(P + 7) // 8, shared_d=int(bool(shared)),
-                                   bexp=mfma_bexp(it.get("amax", None), it["a64"]),
+                                   bexp=mfma_bexp(it.get("amax", None), it["a64"]), ptb=ptb,
                                    Mt=_ptr(it["Mt"]), D=_ptr(it["D"]), K=_ptr(it["K"]), a=_ptr(it["a64"]),
                                    Bf=_ptr(ws), x=_ptr(it["x"]))
                 pb, mb = ctypes.c_int(), ctypes.c_int()
@@ -166,7 +176,7 @@ class EvalMfma:
                 prep.append(pb.value)
                 main.append(mb.value)
             buf, dptr, (pp, pm), (tp, tm) = pack(descs, [prep, main], its[0]["x"].device)
-            self.launches.append((buf, dptr, pp, pm, len(descs), tp, tm, (nkb << 4) | (mode << 8)))
+            self.launches.append((buf, dptr, pp, pm, len(descs), tp, tm, (nkb << 4) | (mode << 8) | (ptb << 12)))
 
     def launch(self):
         for x in self.empty:
