@@ -101,9 +101,9 @@ typedef struct qsft_eval_mfma_desc {
     int32_t ptb;            /* p-tiles per workgroup: 5..8 for q in {2, 4} with n_pad = 64,  *
                              * 8 otherwise (same for every instance of a launch)           */
     const int8_t* Mt;       /* [C][b][n_pad]                                                */
-    const int8_t* D;        /* [C][P][n_pad]                                                */
-    const int8_t* K;        /* [S][n_pad]                                                   */
-    const double* a;        /* [S][2] strengths, complex FP64                               */
+    const int8_t* D;        /* [C][P][n_pad], 16-byte aligned                               */
+    const int8_t* K;        /* [S][n_pad], 16-byte aligned                                  */
+    const double* a;        /* [S][2] strengths, complex FP64, 16-byte aligned              */
     uint16_t* Bf;           /* workspace: B fragments, qsft_eval_mfma_workspace_bytes(shared_d ? 1 : C, P, S) */
     void* x;                /* [C][P][q^b] complex FP32 samples out                         */
 } qsft_eval_mfma_desc;
@@ -298,17 +298,19 @@ int qsft_peel_bind(qsft_peel_desc* host_desc, int q, int n, int b, int C, int P,
 int qsft_peel_bin_blocks(const qsft_peel_desc* host_desc);   /* blocks of the per-bin kernels  */
 int qsft_peel_slot_blocks(const qsft_peel_desc* host_desc);  /* blocks of the per-slot kernels */
 
-/* Whole loop: init, max_iter x (detect, peel), finalize -- no host sync. */
+/* Whole loop: init, max_iter x (detect, peel), finalize -- no host sync.
+ * max_p: the largest P over the instances (<= 256); P <= 128 selects the
+ * detection kernel that holds two column rows per lane instead of four. */
 int qsft_peel_run(const qsft_peel_desc* dev_descs, const int32_t* dev_pre_bins,
                   const int32_t* dev_pre_slots, int n_inst, int blocks_bins, int blocks_slots,
-                  int max_iter, void* stream);
+                  int max_p, int max_iter, void* stream);
 /* The same steps one at a time, for a host-driven loop (the plain path).
  * qsft_peel_init clears the tables and screens every bin of the first pass
  * (|col|^2 > cutoff * P): only non-zeroton bins are stamped for pass 0. */
 int qsft_peel_init(const qsft_peel_desc* dev_descs, const int32_t* dev_pre_bins, const int32_t* dev_pre_slots,
                    int n_inst, int blocks_bins, int blocks_slots, void* stream);
 int qsft_peel_iteration(const qsft_peel_desc* dev_descs, const int32_t* dev_pre_bins, int n_inst,
-                        int blocks_bins, int iteration, void* stream);
+                        int blocks_bins, int max_p, int iteration, void* stream);
 int qsft_peel_finalize(const qsft_peel_desc* dev_descs, const int32_t* dev_pre_slots, int n_inst,
                        int blocks_slots, void* stream);
 

@@ -106,9 +106,10 @@ SIGNATURES = {
                                       ctypes.c_int]),
     "qsft_peel_bin_blocks": (ctypes.c_int, [ctypes.POINTER(PeelDesc)]),
     "qsft_peel_slot_blocks": (ctypes.c_int, [ctypes.POINTER(PeelDesc)]),
-    "qsft_peel_run": (ctypes.c_int, [vp, vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp]),
+    "qsft_peel_run": (ctypes.c_int, [vp, vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                     vp]),
     "qsft_peel_init": (ctypes.c_int, [vp, vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp]),
-    "qsft_peel_iteration": (ctypes.c_int, [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp]),
+    "qsft_peel_iteration": (ctypes.c_int, [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp]),
     "qsft_peel_finalize": (ctypes.c_int, [vp, vp, ctypes.c_int, ctypes.c_int, vp]),
     "qsft_peel_pack_header_bytes": (ctypes.c_size_t, [ctypes.c_int]),
     "qsft_peel_pack": (ctypes.c_int, [vp, ctypes.c_int, vp, ctypes.c_int64, ctypes.c_int, vp]),

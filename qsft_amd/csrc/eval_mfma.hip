@@ -119,20 +119,36 @@ __global__ __launch_bounds__(256) void k_eval_mfma_prep(const qsft_eval_mfma_des
     const int p = 8 * pt + pl;
     const ModQ mq((uint32_t)q);
     const int bexp = d.bexp;
+    // <d_p, k_s> for the thread's four supports: 16-byte loads of the delay row
+    // and of the four digit rows (rows past P or S re-read the last one and
+    // are zeroed below)
+    const int4* __restrict__ dr = reinterpret_cast<const int4*>(d.D + ((size_t)c * d.P + min(p, d.P - 1)) * n_pad);
+    const int4* __restrict__ kr[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+        kr[j] = reinterpret_cast<const int4*>(d.K + (size_t)min(16 * sb + 4 * g + j, d.S - 1) * n_pad);
+    int acc[4] = {0, 0, 0, 0};
+    for (int w = 0; w < (n_pad >> 4); ++w) {
+        const int4 dv = dr[w];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int4 kv = kr[j][w];
+            acc[j] = __builtin_amdgcn_sdot4(dv.x, kv.x, acc[j], false);
+            acc[j] = __builtin_amdgcn_sdot4(dv.y, kv.y, acc[j], false);
+            acc[j] = __builtin_amdgcn_sdot4(dv.z, kv.z, acc[j], false);
+            acc[j] = __builtin_amdgcn_sdot4(dv.w, kv.w, acc[j], false);
+        }
+    }
     uint32_t lr[4][kLimbs], li[4][kLimbs];
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
         const int s = 16 * sb + 4 * g + j;
         double br = 0.0, bi = 0.0;
         if (s < d.S && p < d.P) {
-            const int* __restrict__ dr = reinterpret_cast<const int*>(d.D + ((size_t)c * d.P + p) * n_pad);
-            const int* __restrict__ kr = reinterpret_cast<const int*>(d.K + (size_t)s * n_pad);
-            int acc = 0;
-            for (int w = 0; w < (n_pad >> 2); ++w) acc = __builtin_amdgcn_sdot4(dr[w], kr[w], acc, false);
-            const double2 wv = tab[mq((uint32_t)acc)];
-            const double ar = d.a[2 * s], ai = d.a[2 * s + 1];
-            br = ldexp(ar * wv.x - ai * wv.y, bexp);
-            bi = ldexp(ar * wv.y + ai * wv.x, bexp);
+            const double2 wv = tab[mq((uint32_t)acc[j])];
+            const double2 av = reinterpret_cast<const double2*>(d.a)[s];
+            br = ldexp(av.x * wv.x - av.y * wv.y, bexp);
+            bi = ldexp(av.x * wv.y + av.y * wv.x, bexp);
         }
         split2(br, lr[j]);
         split2(bi, li[j]);
@@ -442,7 +458,8 @@ static bool mfma_desc_ok(const qsft_eval_mfma_desc* h) {
     return h && h->q >= 2 && h->q <= QSFT_MAX_Q && h->b >= 1 && h->b <= QSFT_MAX_B && h->n >= 1 && h->n <= h->n_pad &&
            h->n_pad % 64 == 0 && h->n_pad <= QSFT_MAX_N && h->C >= 1 && h->P >= 1 && h->S >= 1 &&
            h->nsb == (h->S + 15) / 16 && h->npt == (h->P + 7) / 8 && (h->shared_d == 0 || h->shared_d == 1) &&
-           h->bexp >= -100 && h->bexp <= 100 && h->ptb >= 1 && h->ptb <= kPtPerBlock;
+           h->bexp >= -100 && h->bexp <= 100 && h->ptb >= 1 && h->ptb <= kPtPerBlock &&
+           (((uintptr_t)h->D | (uintptr_t)h->K | (uintptr_t)h->a) & 15) == 0;
 }
 
 extern "C" size_t qsft_eval_mfma_workspace_bytes(int C, int P, int S) {

@@ -21,24 +21,27 @@
 // xor-butterfly reductions, bit-identical on every lane) and coordinates
 // i = lane + 1 (+ 64) for nso1 (one atan2 per lane).
 //
-// Scan stamps: a streaming screen stamps the non-zeroton bins for pass 0; a
-// later pass scans only the bins that were non-zeroton on the previous pass or
-// were peeled into by it (an untouched zeroton keeps its column, so it stays
-// a zeroton). Both per-pass kernels mark such a bin with a plain store of
-// pass + 1 (every writer stores the same value), and the next pass's waves
-// test the stamps of their 64 bins with one load and a ballot: no list, no
-// returning atomic per bin.
+// Scan stamps: a screen stamps the non-zeroton bins for pass 0; a later pass
+// is due on the bins that were non-zeroton on the previous pass or were
+// peeled into by it (an untouched zeroton keeps its column, so it stays a
+// zeroton). The per-pass kernels mark such a bin with a plain store of
+// pass + 1 (every writer of one bin in one kernel stores the same value),
+// and the next pass's waves test the stamps of their 64 bins with one load
+// and a ballot: no list, no returning atomic per bin. A multiton carries an
+// extra bit until a ball is queued on it; a due bin with that bit has the
+// column it was a multiton with, so it is counted as one without being read.
 //
-// Kernel A (detect) registers each singleton in an open-addressing hash
-// table keyed by a 64-bit hash ("tag") of k: CAS claims a slot, atomicMin /
-// atomicMax on (pass+1) << 32 | bin record the first and the last detection,
-// and a record (time, rho) is pushed on the slot's lock-free list; a wave's
-// singletons are registered together, one lane each. Kernel B (peel) checks
-// that the detection's k equals its slot's key (a 64-bit tag collision sets
-// an error bit instead of merging two keys), writes the key of new slots, and
-// lets the last detection of each key peel it from every group with
-// coalesced FP64 atomics over the contiguous column. No kernel waits on
-// another thread's progress.
+// Kernel A (detect) first subtracts the balls queued on the bin by the
+// previous pass and writes the column back, then classifies it and registers
+// each singleton in an open-addressing hash table keyed by a 64-bit hash
+// ("tag") of k: CAS claims a slot, atomicMin / atomicMax on
+// (pass+1) << 32 | bin record the first and the last detection, and a record
+// (time, rho) is pushed on the slot's lock-free list; a wave's singletons are
+// registered together, one lane each. Kernel B (apply) checks that the
+// detection's k equals its slot's key (a 64-bit tag collision sets an error
+// bit instead of merging two keys), writes the key of new slots, and lets the
+// last detection of each key queue the ball on its bin in every group. No
+// kernel waits on another thread's progress.
 #include "qsft_common.h"
 
 namespace qsft {
@@ -49,12 +52,7 @@ constexpr int kBinsPerWave = 64;      // consecutive bins owned by one wave (one
 constexpr int kRowsPerLane = kColRowsPerLane;   // P <= 256
 constexpr int kSlotThreads = 256;
 constexpr uint32_t kNil = 0xFFFFFFFFu;
-
-__device__ __forceinline__ int wave_sum_i(int v) {
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
-    return v;
-}
+constexpr int kHeldMulti = 1 << 30;   // mark bit: multiton when last scanned, no ball queued on it since
 
 // numpy's remainder(x, 2 pi) for finite x: fmod, then shift negatives up.
 __device__ __forceinline__ double rem_2pi(double x) {
@@ -70,12 +68,12 @@ __device__ __forceinline__ double rem_2pi(double x) {
 
 // <row, k> over nw 4-digit words (nw a multiple of 16, rows 16-byte aligned);
 // k is the wave's LDS copy (broadcast reads), both read 16 bytes at a time.
-__device__ __forceinline__ uint32_t dot_row(const int8_t* __restrict__ row, const uint32_t* k, int nw) {
-    const int4* __restrict__ r = reinterpret_cast<const int4*>(row);
+__device__ __forceinline__ uint32_t dot_row(const QSFT_GLOBAL int8_t* __restrict__ row, const uint32_t* k, int nw) {
+    const QSFT_GLOBAL gi4* __restrict__ r = reinterpret_cast<const QSFT_GLOBAL gi4*>(row);
     const uint4* k4 = reinterpret_cast<const uint4*>(k);
     int acc = 0;
     for (int w = 0; w < (nw >> 2); ++w) {
-        const int4 rv = r[w];
+        const gi4 rv = r[w];
         const uint4 kv = k4[w];
         acc = __builtin_amdgcn_sdot4(rv.x, (int)kv.x, acc, false);
         acc = __builtin_amdgcn_sdot4(rv.y, (int)kv.y, acc, false);
@@ -107,33 +105,32 @@ __device__ __forceinline__ uint32_t dot_row_lds(const int8_t* row, const uint32_
 constexpr int kStageOffBytes = 2048;
 constexpr int kStageMtBytes = 4096;
 
-__device__ __forceinline__ const int8_t* stage_offsets(const qsft_peel_desc& d, int8_t* buf) {
+__device__ __forceinline__ bool stage_offsets(const qsft_peel_desc& d, int8_t* buf) {
     const int R = d.P / d.p1, n_pad = d.n_pad;
     if (!(d.flags & QSFT_PEEL_UNIT_D) || !(d.flags & QSFT_PEEL_SHARED_D) || R * n_pad > kStageOffBytes)
-        return nullptr;
+        return false;
     const int words = R * n_pad / 16;
     for (int e = threadIdx.x; e < words; e += blockDim.x) {
         const int r = e / (n_pad / 16), w = e - r * (n_pad / 16);
-        reinterpret_cast<uint4*>(buf)[e] = reinterpret_cast<const uint4*>(d.D + (size_t)r * d.p1 * n_pad)[w];
+        reinterpret_cast<gu4*>(buf)[e] = reinterpret_cast<const QSFT_GLOBAL gu4*>(gptr(d.D) + (size_t)r * d.p1 * n_pad)[w];
     }
-    return buf;
+    return true;
 }
 
-__device__ __forceinline__ const int8_t* stage_mt(const qsft_peel_desc& d, int8_t* buf) {
+__device__ __forceinline__ bool stage_mt(const qsft_peel_desc& d, int8_t* buf) {
     const int bytes = d.C * d.b * d.n_pad;
-    if (bytes > kStageMtBytes) return d.Mt;
+    if (bytes > kStageMtBytes) return false;
     for (int e = threadIdx.x; e < bytes / 16; e += blockDim.x)
-        reinterpret_cast<uint4*>(buf)[e] = reinterpret_cast<const uint4*>(d.Mt)[e];
-    return buf;
+        reinterpret_cast<gu4*>(buf)[e] = reinterpret_cast<const QSFT_GLOBAL gu4*>(gptr(d.Mt))[e];
+    return true;
 }
 
-__device__ __forceinline__ uint64_t hash_words(const uint32_t* k, int nw) {
-    uint64_t h = 0xCBF29CE484222325ull;
-    for (int w = 0; w < nw; ++w) {
-        h ^= k[w];
-        h *= 0x100000001B3ull;
-    }
-    return mix64(h);
+// 64-bit hash of the wave's k (words kk[0 .. nw)): lane w mixes word w with
+// its position, the lanes' values are XOR-combined and mixed again. Every
+// lane returns the same value; equal k give equal hashes.
+__device__ __forceinline__ uint64_t hash_k(const uint32_t* kk, int nw, int lane) {
+    const uint64_t h = (lane < nw) ? mix64((uint64_t)kk[lane] ^ ((uint64_t)(lane + 1) * 0x9E3779B97F4A7C15ull)) : 0ull;
+    return mix64(wave_xor64(h) ^ 0xCBF29CE484222325ull);
 }
 
 __device__ __forceinline__ void fill_wtable_d(double2* tab, int q) {
@@ -145,10 +142,15 @@ __device__ __forceinline__ void fill_wtable_d(double2* tab, int q) {
 }
 
 // LDS writes by one lane, read by other lanes of the same wave.
+// A wave's LDS operations execute in order, so this only waits for the wave's
+// outstanding LDS (and scalar-memory) operations and keeps the compiler from
+// moving memory accesses across it; vector-memory loads in flight (the column
+// prefetch) are not waited for.
 __device__ __forceinline__ void wave_lds_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __asm__ volatile("" ::: "memory");
+    __builtin_amdgcn_s_waitcnt(0xF | (0x3 << 14) | (0x7 << 4));   // lgkmcnt(0)
     __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    __asm__ volatile("" ::: "memory");
 }
 
 __global__ __launch_bounds__(kSlotThreads) void k_peel_init(const qsft_peel_desc* __restrict__ descs,
@@ -179,13 +181,6 @@ __device__ __forceinline__ void load_col(double2 (&u)[kRowsPerLane], const doubl
         const int p = lane + 64 * r;
         u[r] = (p < P) ? col[p] : make_double2(0.0, 0.0);
     }
-}
-
-// q^(b - 1 - dg): weight of digit dg (MSB first) in a bin index
-__device__ __forceinline__ int digit_weight(int q, int b, int dg) {
-    int pw = 1;
-    for (int tt = dg + 1; tt < b; ++tt) pw *= q;
-    return pw;
 }
 
 // First-pass screen. Wave w owns bins [64 w, 64 w + 64); lane L stamps bin
@@ -235,9 +230,10 @@ __global__ __launch_bounds__(kBinThreads) void k_peel_screen(const qsft_peel_des
 // row p = r p1 + i is offset_r + s e_(i-1), so <row, k> = <offset_r, k> + s k_(i-1)
 // (i > 0): lanes r < R form the R offset products, every row takes one of them
 // and one digit. Otherwise each row is a full dot product.
-__device__ __forceinline__ void row_phases(const qsft_peel_desc& d, const int8_t* __restrict__ Dc,
+template <int NR>
+__device__ __forceinline__ void row_phases(const qsft_peel_desc& d, const QSFT_GLOBAL int8_t* __restrict__ Dc,
                                            const uint32_t* kk, const double2* tab, const ModQ& mq, int lane,
-                                           double2 (&w)[kRowsPerLane], const int8_t* offs = nullptr) {
+                                           double2 (&w)[NR], const int8_t* offs, bool offs_staged) {
     const int P = d.P, n_pad = d.n_pad, nw = n_pad >> 2;
     if (d.flags & QSFT_PEEL_UNIT_D) {
         const int p1 = d.p1, R = P / p1, q = d.q;
@@ -246,15 +242,25 @@ __device__ __forceinline__ void row_phases(const qsft_peel_desc& d, const int8_t
         const uint32_t m1 = (uint32_t)(0xFFFFFFFFull / (uint32_t)p1) + 1u;   // p / p1 exact for p < 256
         int doff = 0;
         if (lane < R)                                              // offset rows: LDS copy when staged
-            doff = (int)mq(offs ? dot_row_lds(offs + (size_t)lane * n_pad, kk, nw)
+            doff = (int)mq(offs_staged ? dot_row_lds(offs + (size_t)lane * n_pad, kk, nw)
                                 : dot_row(Dc + (size_t)lane * p1 * n_pad, kk, nw));
+        // up to 4 offsets as wave-uniform values (selected per row), more by a shuffle per row
+        const bool few = R <= 4;
+        int o0 = 0, o1 = 0, o2 = 0, o3 = 0;
+        if (few) {
+            o0 = __builtin_amdgcn_readlane(doff, 0);
+            o1 = __builtin_amdgcn_readlane(doff, 1);
+            o2 = __builtin_amdgcn_readlane(doff, 2);
+            o3 = __builtin_amdgcn_readlane(doff, 3);
+        }
 #pragma unroll
-        for (int r = 0; r < kRowsPerLane; ++r) {
+        for (int r = 0; r < NR; ++r) {
             const int p = lane + 64 * r;
             const uint32_t pc = (uint32_t)min(p, P - 1);
             const int blk = (int)__umulhi(pc, m1);
             const int i = (int)pc - blk * p1;
-            const int base = __shfl(doff, blk);                    // every lane takes part
+            const int base = few ? (blk == 0 ? o0 : blk == 1 ? o1 : blk == 2 ? o2 : o3)
+                                 : __shfl(doff, blk);              // every lane takes part
             w[r] = make_double2(0.0, 0.0);
             if (p < P) {
                 int e = base;
@@ -265,56 +271,88 @@ __device__ __forceinline__ void row_phases(const qsft_peel_desc& d, const int8_t
         }
     } else {
 #pragma unroll
-        for (int r = 0; r < kRowsPerLane; ++r) {
+        for (int r = 0; r < NR; ++r) {
             const int p = lane + 64 * r;
             w[r] = (p < P) ? tab[mq(dot_row(Dc + (size_t)p * n_pad, kk, nw))] : make_double2(0.0, 0.0);
         }
     }
 }
 
-// A queued ball: the entry index and, fetched ahead, its k word for this lane
-// (lanes < n_pad / 4), its rho and the next entry of the queue.
+// A queued ball: its entry, the next entry of its queue and its rho (scalar
+// loads: the previous pass's arrays are not written by this kernel). Its k row
+// goes into an LDS buffer of the wave, either by an LDS-direct load issued a
+// bin ahead (prefetch_krow, waited for with the column) or by plain loads
+// (load_krow_now).
 struct Pending {
     int ent, next;
-    uint32_t kw;
     double rx, ry;
 };
 
-__device__ __forceinline__ Pending fetch_pending(const qsft_peel_desc& d, int ent, int from, int lane) {
-    ent = __builtin_amdgcn_readfirstlane(ent);                     // wave-uniform: scalar loads below
-    Pending pd{ent, -1, 0u, 0.0, 0.0};
+__device__ __forceinline__ Pending pending_head(const qsft_peel_desc& d, int ent, int from) {
+    ent = __builtin_amdgcn_readfirstlane(ent);                     // wave-uniform
+    Pending pd{ent, -1, 0.0, 0.0};
     if (ent < 0) return pd;
-    const int C = d.C, nw = d.n_pad >> 2;
-    const size_t CQ = (size_t)C * d.Q;
-    const int sbin = ent / C;                                      // the ball's detection bin
-    const int8_t* __restrict__ kprev = d.kbin + (size_t)(from & 1) * CQ * d.n_pad;
-    const double* __restrict__ rprev = d.rho_bin + (size_t)(from & 1) * CQ * 2;
-    if (lane < nw) pd.kw = reinterpret_cast<const uint32_t*>(kprev + (size_t)sbin * d.n_pad)[lane];
+    const size_t CQ = (size_t)d.C * d.Q;
+    const int sbin = ent / d.C;                                    // the ball's detection bin
+    const QSFT_CONST double* rprev = cptr(d.rho_bin) + (size_t)(from & 1) * CQ * 2;
     pd.rx = rprev[2 * (size_t)sbin];
     pd.ry = rprev[2 * (size_t)sbin + 1];
-    pd.next = d.pend_next[ent];
+    pd.next = cptr(d.pend_next)[ent];
     return pd;
 }
 
+__device__ __forceinline__ const int8_t* krow_src(const qsft_peel_desc& d, int ent, int from) {
+    const size_t CQ = (size_t)d.C * d.Q;
+    return d.kbin + (size_t)(from & 1) * CQ * d.n_pad + (size_t)(ent / d.C) * d.n_pad;
+}
+
+// LDS copy of a queued ball, loaded a bin ahead by LDS-direct loads (the
+// caller waits for the vector-memory counter): its k row (16 bytes per lane),
+// its rho and the next entry of its queue.
+struct PendingSlot {
+    uint32_t krow[QSFT_MAX_N / 4];
+    double2 rho;
+    int32_t next;
+    int32_t pad[3];
+};
+
+__device__ __forceinline__ void prefetch_pending(const qsft_peel_desc& d, int ent, int from, int lane, PendingSlot* ps) {
+    if (lane < (d.n_pad >> 4))
+        dma16(krow_src(d, ent, from), (uint32_t)lane * 16u, __builtin_amdgcn_readfirstlane(lds_addr(ps->krow)));
+    if (lane == 0) {
+        const size_t CQ = (size_t)d.C * d.Q;
+        dma16(d.rho_bin + (size_t)(from & 1) * CQ * 2 + 2 * (size_t)(ent / d.C), 0u,
+              __builtin_amdgcn_readfirstlane(lds_addr(&ps->rho)));
+        dma4(d.pend_next + ent, 0u, __builtin_amdgcn_readfirstlane(lds_addr(&ps->next)));
+    }
+}
+
+__device__ __forceinline__ void load_krow_now(const qsft_peel_desc& d, int ent, int from, int lane, uint32_t* krow) {
+    if (lane < (d.n_pad >> 2))
+        krow[lane] = reinterpret_cast<const QSFT_GLOBAL uint32_t*>(gptr(krow_src(d, ent, from)))[lane];
+    wave_lds_sync();
+}
+
 // Subtract from bin t's column u (rows p = lane + 64 r, held by the wave) every
-// ball queued on it by pass `from` (first entry `pd`, already fetched):
-// rho w^{<D_c row, k>} with the ball's k and rho from that pass's singleton
-// arrays; write the column back and clear the queue. The queue order comes
-// from an atomic race, so the first kSortedPending entries are applied in
-// ascending entry order (the FP64 result does not depend on the race); more
-// balls than that on one bin in one pass follow in queue order.
+// ball queued on it by pass `from`: rho w^{<D_c row, k>} with the ball's k
+// and rho from that pass's singleton arrays (the caller writes the column
+// back with store_col). The first ball of the queue (pd) has its k row in kfirst; the
+// others are loaded into kk. The queue order comes from an atomic race, so the
+// first kSortedPending entries are applied in ascending entry order (the FP64
+// result does not depend on the race); more balls than that on one bin in one
+// pass follow in queue order.
 constexpr int kSortedPending = 8;
 
-__device__ __forceinline__ void subtract_ball(const qsft_peel_desc& d, const int8_t* __restrict__ Dc,
-                                              const Pending& pd, double2 (&u)[kRowsPerLane], uint32_t* kk,
-                                              const double2* tab, const ModQ& mq, int lane, const int8_t* offs) {
-    const int nw = d.n_pad >> 2, P = d.P;
-    if (lane < nw) kk[lane] = pd.kw;
-    wave_lds_sync();
-    double2 w[kRowsPerLane];
-    row_phases(d, Dc, kk, tab, mq, lane, w, offs);
+template <int NR>
+__device__ __forceinline__ void subtract_ball(const qsft_peel_desc& d, const QSFT_GLOBAL int8_t* __restrict__ Dc,
+                                              const Pending& pd, double2 (&u)[NR], const uint32_t* krow,
+                                              const double2* tab, const ModQ& mq, int lane, const int8_t* offs,
+                                              bool offs_staged) {
+    const int P = d.P;
+    double2 w[NR];
+    row_phases(d, Dc, krow, tab, mq, lane, w, offs, offs_staged);
 #pragma unroll
-    for (int r = 0; r < kRowsPerLane; ++r) {
+    for (int r = 0; r < NR; ++r) {
         const int p = lane + 64 * r;
         if (p < P) {
             const double sx = __dsub_rn(__dmul_rn(pd.rx, w[r].x), __dmul_rn(pd.ry, w[r].y));
@@ -323,22 +361,25 @@ __device__ __forceinline__ void subtract_ball(const qsft_peel_desc& d, const int
             u[r].y = __dsub_rn(u[r].y, sy);
         }
     }
-    wave_lds_sync();                                               // kk is rewritten next
+    wave_lds_sync();                                               // the k row buffer is rewritten next
 }
 
+template <int NR>
 __device__ __forceinline__ void apply_pending(const qsft_peel_desc& d, int t, int c, int from, Pending pd,
-                                              double2 (&u)[kRowsPerLane], uint32_t* kk, const double2* tab,
-                                              const ModQ& mq, int lane, const int8_t* offs) {
+                                              double2 (&u)[NR], const uint32_t* kfirst, uint32_t* kk,
+                                              const double2* tab, const ModQ& mq, int lane, const int8_t* offs,
+                                              bool offs_staged) {
     if (pd.ent < 0) return;
     const int P = d.P, n_pad = d.n_pad;
-    const int8_t* __restrict__ Dc = d.D + (size_t)((d.flags & QSFT_PEEL_SHARED_D) ? 0 : c) * P * n_pad;
+    const QSFT_GLOBAL int8_t* __restrict__ Dc =
+        gptr(d.D) + (size_t)((d.flags & QSFT_PEEL_SHARED_D) ? 0 : c) * P * n_pad;
     // lane a holds the a-th queued entry (the first kSortedPending of them)
     int mine = (lane == 0) ? pd.ent : -1;
-    int m = 1, rest = pd.next;
+    int m = 1, rest = __builtin_amdgcn_readfirstlane(pd.next);
     while (rest >= 0 && m < kSortedPending) {
         if (lane == m) mine = rest;
         ++m;
-        rest = __builtin_amdgcn_readfirstlane(d.pend_next[rest]);
+        rest = __builtin_amdgcn_readfirstlane(cptr(d.pend_next)[rest]);
     }
     int rank = 0;                                                  // position in ascending order
     for (int a = 0; a < m; ++a) {
@@ -346,25 +387,37 @@ __device__ __forceinline__ void apply_pending(const qsft_peel_desc& d, int t, in
         if (lane < m && o < mine) ++rank;                          // entries are distinct
     }
     for (int a = 0;; ++a) {
-        Pending pa;
+        int e;
         if (a < m) {
-            const int e = __shfl(mine, __builtin_ctzll(__ballot(lane < m && rank == a)));
-            pa = (e == pd.ent) ? pd : fetch_pending(d, e, from, lane);
+            e = __shfl(mine, __builtin_ctzll(__ballot(lane < m && rank == a)));
         } else if (rest >= 0) {                                    // beyond kSortedPending: queue order
-            pa = fetch_pending(d, rest, from, lane);
-            rest = pa.next;
+            e = rest;
         } else {
             break;
         }
-        subtract_ball(d, Dc, pa, u, kk, tab, mq, lane, offs);
+        e = __builtin_amdgcn_readfirstlane(e);
+        if (e == pd.ent) {
+            subtract_ball(d, Dc, pd, u, kfirst, tab, mq, lane, offs, offs_staged);
+        } else {
+            const Pending pa = pending_head(d, e, from);
+            if (a >= m) rest = pa.next;
+            load_krow_now(d, e, from, lane, kk);
+            subtract_ball(d, Dc, pa, u, kk, tab, mq, lane, offs, offs_staged);
+        }
     }
-    double2* __restrict__ col = reinterpret_cast<double2*>(d.U) + (size_t)t * P;
+}
+
+// Write bin t's column back after its queue was applied, and clear the queue.
+template <int NR>
+__device__ __forceinline__ void store_col(const qsft_peel_desc& d, int t, const double2 (&u)[NR], int lane) {
+    const int P = d.P;
+    QSFT_GLOBAL gd2* __restrict__ col = reinterpret_cast<QSFT_GLOBAL gd2*>(gptr(d.U)) + (size_t)t * P;
 #pragma unroll
-    for (int r = 0; r < kRowsPerLane; ++r) {
+    for (int r = 0; r < NR; ++r) {
         const int p = lane + 64 * r;
-        if (p < P) col[p] = u[r];
+        if (p < P) col[p] = gd2{u[r].x, u[r].y};
     }
-    if (lane == 0) d.pend_head[t] = -1;                            // the queue is consumed
+    if (lane == 0) gptr(d.pend_head)[t] = -1;                      // the queue is consumed
 }
 
 // After the last pass: apply what its kernel B queued, so U ends as the
@@ -374,13 +427,14 @@ __global__ __launch_bounds__(kSlotThreads) void k_peel_flush(const qsft_peel_des
                                                              const int32_t* __restrict__ pre, int n_inst) {
     __shared__ double2 tab[QSFT_MAX_Q];
     __shared__ __align__(16) uint32_t kws[kSlotThreads / 64][QSFT_MAX_N / 4];
+    __shared__ __align__(16) uint32_t kfirst[kSlotThreads / 64][QSFT_MAX_N / 4];
     const int inst = find_inst(pre, n_inst, blockIdx.x);
     const qsft_peel_desc& d = descs[inst];
     __shared__ __align__(16) int8_t offbuf[kStageOffBytes];
     const uint32_t iters = d.ctr[QSFT_CTR_ITERS];
     if (iters == 0) return;
     fill_wtable_d(tab, d.q);
-    const int8_t* offs = stage_offsets(d, offbuf);
+    const bool staged = stage_offsets(d, offbuf);
     __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int CQ = d.C * d.Q, P = d.P;
@@ -393,9 +447,82 @@ __global__ __launch_bounds__(kSlotThreads) void k_peel_flush(const qsft_peel_des
         todo &= todo - 1;
         double2 u[kRowsPerLane];
         load_col(u, reinterpret_cast<const double2*>(d.U) + (size_t)t * P, P, lane);
-        const Pending pd = fetch_pending(d, d.pend_head[t], (int)iters - 1, lane);
-        apply_pending(d, t, t / d.Q, (int)iters - 1, pd, u, kws[wave], tab, mq, lane, offs);
+        const Pending pd = pending_head(d, d.pend_head[t], (int)iters - 1);
+        load_krow_now(d, pd.ent, (int)iters - 1, lane, kfirst[wave]);
+        apply_pending(d, t, t / d.Q, (int)iters - 1, pd, u, kfirst[wave], kws[wave], tab, mq, lane, offbuf, staged);
+        store_col(d, t, u, lane);
     }
+}
+
+// Digit of a phase: the k of the nearest multiple of 2 pi / q, as the exact
+// FP64 rules below compute it, taken from a cheaper test wherever the phase is
+// not within a small margin of a decision boundary (half-way between two
+// multiples). For q = 4 and q = 2 the sector follows from comparisons of the
+// components; otherwise an FP32 atan2 of the components, scaled by a power of
+// two into FP32 range, is off by < 1e-6 rad, far inside the margin (1e-4 of a
+// sector, >= 1e-5 rad for q <= 64).
+// Returns -1 where the exact rule must decide (near a boundary, zero, inf, NaN).
+__device__ __forceinline__ int fast_sector(double zr, double zi, int q) {
+    const double ax = fabs(zr), ay = fabs(zi);
+    if (!(ax < 1.0e300 && ay < 1.0e300)) return -1;               // inf or NaN
+    const double m = fmax(ax, ay);
+    if (q == 4) {
+        if (!(fabs(ax - ay) > 1e-9 * m)) return -1;
+        return ax > ay ? (zr > 0.0 ? 0 : 2) : (zi > 0.0 ? 1 : 3);
+    }
+    if (q == 2) {
+        if (!(ax > 1e-9 * m)) return -1;
+        return zr > 0.0 ? 0 : 1;
+    }
+    if (!(m > 1.0e-300)) return -1;
+    const int ex = ilogb(m);
+    const float fx = (float)ldexp(zr, -ex), fy = (float)ldexp(zi, -ex);
+    float v = atan2f(fy, fx) * ((float)q * 0.15915494309189535f);   // q / (2 pi)
+    if (v < 0.0f) v += (float)q;
+    const float r = rintf(v);
+    if (!(fabsf(v - r) < 0.5f - 1.0e-4f)) return -1;
+    int k = (int)r;
+    if (k >= q) k -= q;
+    return k;
+}
+
+// nso1 digit of coordinate i (reference: the nso1 branch of
+// singleton_detection, qsft/reconstruct.py:12-31): the phase of the mean over
+// the R repeats of u_{r,0} conj(u_{r,i}), rounded to the nearest 2 pi t / q.
+__device__ __forceinline__ uint32_t nso1_digit(double zr, double zi, int q, int R) {
+    const int f = fast_sector(zr, zi, q);
+    if (f >= 0) return (uint32_t)f;
+    zr = __ddiv_rn(zr, (double)R);
+    zi = __ddiv_rn(zi, (double)R);
+    const double twopi = 6.283185307179586;
+    const double th = rem_2pi(atan2(zi, zr));
+    const double step = twopi / (double)q;                         // 2 * np.pi / q
+    int best = 0;
+    double bestv = fabs(__dsub_rn(0.0, th));
+    for (int tt = 1; tt <= q; ++tt) {
+        const double v = fabs(__dsub_rn(__dmul_rn(step, (double)tt), th));
+        if (v < bestv) { bestv = v; best = tt; }
+    }
+    return (uint32_t)(best % q);
+}
+
+// Noiseless digit of coordinate i (reference: qsft/reconstruct.py:100-113):
+// round(q (angle(u_i) - angle(u_0)) / (2 pi)) mod q. The phase difference is
+// the phase of u_i conj(u_0), exact to ~1e-16 rad when both magnitudes are
+// far from the FP64 range ends.
+__device__ __forceinline__ uint32_t noiseless_digit(double2 v0, double2 vi, int q) {
+    const double m0 = fmax(fabs(v0.x), fabs(v0.y)), mi = fmax(fabs(vi.x), fabs(vi.y));
+    if (m0 > 1.0e-150 && m0 < 1.0e150 && mi > 1.0e-150 && mi < 1.0e150) {
+        const double wr = vi.x * v0.x + vi.y * v0.y, wi = vi.y * v0.x - vi.x * v0.y;
+        const int f = fast_sector(wr, wi, q);
+        if (f >= 0) return (uint32_t)f;
+    }
+    const double twopi = 6.283185307179586;
+    const double a0 = atan2(v0.y, v0.x), ai = atan2(vi.y, vi.x);
+    const double v = __ddiv_rn(__dmul_rn((double)q, __dsub_rn(ai, a0)), twopi);
+    long long kq = (long long)rint(v) % q;
+    if (kq < 0) kq += q;
+    return (uint32_t)kq;
 }
 
 // Kernel A. Wave w of an instance owns bins [64 w, 64 w + 64): it classifies
@@ -405,11 +532,13 @@ __global__ __launch_bounds__(kSlotThreads) void k_peel_flush(const qsft_peel_des
 // hash table, so the atomics of up to 16 registrations are in flight
 // together, and the pass counters take one atomic per wave. Three waves per
 // SIMD fit without scratch (the FP64 atan2 is the largest register user).
+template <int NR>
 __global__ __launch_bounds__(kBinThreads) __attribute__((amdgpu_waves_per_eu(3))) void k_peel_detect(const qsft_peel_desc* __restrict__ descs,
                                                              const int32_t* __restrict__ pre, int n_inst, int it) {
     __shared__ double2 tab[QSFT_MAX_Q];
     __shared__ __align__(16) uint32_t kws[kWaves][QSFT_MAX_N / 4];
-    __shared__ __align__(16) double2 colw[kWaves][2][64 * kRowsPerLane];   // current / next column
+    __shared__ __align__(16) PendingSlot pslot[kWaves][2];                // first queued ball, per column slot
+    __shared__ __align__(16) double2 colw[kWaves][2][64 * NR];   // current / next column
     __shared__ uint64_t sb_h[kWaves][kBinsPerWave];
     __shared__ double2 sb_rho[kWaves][kBinsPerWave];
     __shared__ int32_t sb_t[kWaves][kBinsPerWave];
@@ -422,79 +551,106 @@ __global__ __launch_bounds__(kBinThreads) __attribute__((amdgpu_waves_per_eu(3))
     if (ctr[it * QSFT_CTR_ITER_STRIDE + 3] == 0) return;          // loop already stopped
     const int q = d.q, n = d.n, b = d.b, C = d.C, P = d.P, Q = d.Q, n_pad = d.n_pad;
     const int nw = n_pad >> 2;
-    fill_wtable_d(tab, q);
-    const int8_t* offs = stage_offsets(d, offbuf);
-    const int8_t* mtab = stage_mt(d, mtbuf);
-    __syncthreads();
-
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int CQ = C * Q;
     const int base = ((blockIdx.x - pre[inst]) * kWaves + wave) * kBinsPerWave;
-    if (base >= CQ) return;
-    const bool want = base + lane < CQ && d.mark[base + lane] == it;
-    uint64_t todo = __ballot(want);
-    if (todo == 0) return;
+    // The wave's bins due this pass. A held multiton (a multiton last pass that
+    // no ball has been queued on since) keeps its column, so it is a multiton
+    // again: it is counted and re-stamped without being read.
+    QSFT_GLOBAL int32_t* __restrict__ mark = gptr(d.mark);
+    const int mk = (base + lane < CQ) ? mark[base + lane] : -1;
+    const bool due = (mk & ~kHeldMulti) == it;
+    const bool held = due && (mk & kHeldMulti) != 0;
+    uint64_t todo = __ballot(due && !held);
+    const uint32_t nheld = (uint32_t)__popcll(__ballot(held));
+    if (held) mark[base + lane] = (it + 1) | kHeldMulti;
+    if (!__syncthreads_or(todo != 0)) {                            // nothing to read in this block
+        if (lane == 0 && nheld) atomicAdd(ctr + it * QSFT_CTR_ITER_STRIDE + 1, nheld);
+        return;
+    }
+    fill_wtable_d(tab, q);
+    const bool offs_staged = stage_offsets(d, offbuf);
+    const bool mt_staged = stage_mt(d, mtbuf);
+    __syncthreads();
+    if (todo == 0) {
+        if (lane == 0 && nheld) atomicAdd(ctr + it * QSFT_CTR_ITER_STRIDE + 1, nheld);
+        return;
+    }
     const uint32_t scanned = (uint32_t)__popcll(todo);
 
     uint32_t* kk = kws[wave];
     int8_t* kb8 = reinterpret_cast<int8_t*>(kk);
     const ModQ mq((uint32_t)q);
     const double thr = d.cutoff * (double)P;
-    const double twopi = 6.283185307179586;
     const double2* __restrict__ Ub = reinterpret_cast<const double2*>(d.U);
-    int8_t* __restrict__ kcur = d.kbin + (size_t)(it & 1) * CQ * n_pad;   // this pass's k and rho
-    double* __restrict__ rcur = d.rho_bin + (size_t)(it & 1) * CQ * 2;
-    int nz = 0, ns = 0;                                            // wave-uniform
+    QSFT_GLOBAL int8_t* __restrict__ kcur = gptr(d.kbin) + (size_t)(it & 1) * CQ * n_pad;   // this pass's k and rho
+    QSFT_GLOBAL double* __restrict__ rcur = gptr(d.rho_bin) + (size_t)(it & 1) * CQ * 2;
+    int nz = (int)nheld, ns = 0;                                   // wave-uniform
+    int c = base / Q;                                              // group of the current bin
+    int wdig = 0;                                                  // q^(b - 1 - lane): weight of bin digit `lane`
+    if (lane < b) {
+        wdig = 1;
+        for (int e = lane + 1; e < b; ++e) wdig *= q;
+    }
 
     // queue heads of the wave's bins (balls the previous pass peeled into them)
-    const int qhead = (it > 0 && want) ? d.pend_head[base + lane] : -1;
-    // columns stream through two LDS slots per wave (global->LDS loads, no
-    // registers held): the next bin's column lands while this one is classified
+    const int qhead = (it > 0 && due && !held) ? gptr(d.pend_head)[base + lane] : -1;
+    // Bins stream through two LDS slots per wave: each holds a column and the
+    // first queued ball (k row, rho, next entry), loaded by LDS-direct loads
+    // (no registers held) a bin ahead. A step issues the next bin's loads,
+    // classifies the current bin (no global loads or stores on the common
+    // path), then waits once for every vector-memory operation in flight (the
+    // next bin's data, which has had the whole step to land, and the previous
+    // step's stores) and only then issues its own stores.
     const int nrow = (P + 63) >> 6;                                // loads per column
     const uint32_t rowoff = (uint32_t)min(lane, P - 1) * 16u;     // rows past P re-read row P - 1
-    auto issue_col = [&](int tb, double2* slotp) {
+    auto issue = [&](int tb, int ent, int slot) {
+        if (ent >= 0) prefetch_pending(d, ent, it - 1, lane, &pslot[wave][slot]);
         const double2* cb = Ub + (size_t)tb * P;
         for (int r = 0; r < nrow; ++r)
             dma16(cb, min(64 * r * 16u + rowoff, (uint32_t)(P - 1) * 16u),
-                  __builtin_amdgcn_readfirstlane(lds_addr(slotp + 64 * r)));
-    };
-    auto wait_cols = [&](bool keep_next) {                       // all but the next column's loads done
-        if (!keep_next) wait_vm<0>();
-        else if (nrow == 1) wait_vm<1>();
-        else if (nrow == 2) wait_vm<2>();
-        else if (nrow == 3) wait_vm<3>();
-        else wait_vm<4>();
-        __asm__ volatile("" ::: "memory");
+                  __builtin_amdgcn_readfirstlane(lds_addr(&colw[wave][slot][64 * r])));
     };
     int cur = __builtin_ctzll(todo);
     int cslot = 0;
-    issue_col(base + cur, colw[wave][0]);
-    Pending pn = fetch_pending(d, __shfl(qhead, cur), it - 1, lane);
+    int curent = __builtin_amdgcn_readfirstlane(__shfl(qhead, cur));
+    issue(base + cur, curent, 0);
+    wait_vm<0>();
+    __asm__ volatile("" ::: "memory");
     for (;;) {
         const int t = base + cur;
         double2* cw = colw[wave][cslot];
-        const Pending pd = pn;
+        Pending pd{curent, -1, 0.0, 0.0};
+        if (curent >= 0) {
+            const double2 rv = pslot[wave][cslot].rho;
+            pd.rx = rv.x;
+            pd.ry = rv.y;
+            pd.next = pslot[wave][cslot].next;
+        }
         todo &= todo - 1;
         const bool more = todo != 0;
         const int nxt = more ? __builtin_ctzll(todo) : 0;
-        if (more) issue_col(base + nxt, colw[wave][cslot ^ 1]);
-        wait_cols(more);                                          // this bin's column is in cw
-        double2 u[kRowsPerLane];
+        const int nent = more ? __builtin_amdgcn_readfirstlane(__shfl(qhead, nxt)) : -1;
+        if (more) issue(base + nxt, nent, cslot ^ 1);
+        double2 u[NR];
 #pragma unroll
-        for (int r = 0; r < kRowsPerLane; ++r) {
+        for (int r = 0; r < NR; ++r) {
             const int p = lane + 64 * r;
             u[r] = (p < P) ? cw[p] : make_double2(0.0, 0.0);
         }
-        if (more) pn = fetch_pending(d, __shfl(qhead, nxt), it - 1, lane);   // next bin's first queued ball
 
-        const int c = t / Q, j = t - c * Q;
+        while (t >= (c + 1) * Q) ++c;
+        const int j = t - c * Q;
         // balls queued on this bin by the previous pass
-        apply_pending(d, t, c, it - 1, pd, u, kk, tab, mq, lane, offs);
+        apply_pending(d, t, c, it - 1, pd, u, pslot[wave][cslot].krow, kk, tab, mq, lane, offbuf, offs_staged);
+        int mval = 0;                                              // mark to store (0: none)
+        bool single = false;
+        double rho_x = 0.0, rho_y = 0.0;
         if (bin_energy(u) > thr) {                                 // not a zeroton (wave-uniform)
             ++nz;
             if (pd.ent >= 0) {                                     // the column changed: refresh the LDS copy
 #pragma unroll
-                for (int r = 0; r < kRowsPerLane; ++r) {
+                for (int r = 0; r < NR; ++r) {
                     const int p = lane + 64 * r;
                     if (p < P) cw[p] = u[r];
                 }
@@ -519,25 +675,9 @@ __global__ __launch_bounds__(kBinThreads) __attribute__((amdgpu_waves_per_eu(3))
                             zr = __dadd_rn(zr, pr);
                             zi = __dadd_rn(zi, pi_);
                         }
-                        zr = __ddiv_rn(zr, (double)R);
-                        zi = __ddiv_rn(zi, (double)R);
-                        const double th = rem_2pi(atan2(zi, zr));
-                        const double step = twopi / (double)q;     // 2 * np.pi / q
-                        int best = 0;
-                        double bestv = fabs(__dsub_rn(0.0, th));
-                        for (int tt = 1; tt <= q; ++tt) {
-                            const double v = fabs(__dsub_rn(__dmul_rn(step, (double)tt), th));
-                            if (v < bestv) { bestv = v; best = tt; }
-                        }
-                        digit = (uint32_t)(best % q);
+                        digit = nso1_digit(zr, zi, q, R);
                     } else {
-                        const double2 v0 = cw[0];
-                        const double2 vi = cw[i];
-                        const double a0 = atan2(v0.y, v0.x), ai = atan2(vi.y, vi.x);
-                        const double v = __ddiv_rn(__dmul_rn((double)q, __dsub_rn(ai, a0)), twopi);
-                        long long kq = (long long)rint(v) % q;
-                        if (kq < 0) kq += q;
-                        digit = (uint32_t)kq;
+                        digit = noiseless_digit(cw[0], cw[i], q);
                     }
                     kb8[i - 1] = (int8_t)digit;
                 }
@@ -545,18 +685,18 @@ __global__ __launch_bounds__(kBinThreads) __attribute__((amdgpu_waves_per_eu(3))
             wave_lds_sync();
 
             // --- signature, rho, residual (rows across lanes); the column is
-            // re-read from LDS so its registers are free during the atan2 above
+            // re-read from LDS so its registers are free during the k digits
 #pragma unroll
-            for (int r = 0; r < kRowsPerLane; ++r) {
+            for (int r = 0; r < NR; ++r) {
                 const int p = lane + 64 * r;
                 u[r] = (p < P) ? cw[p] : make_double2(0.0, 0.0);
             }
-            const int8_t* __restrict__ Dc = d.D + (size_t)c * P * n_pad;
-            double2 w[kRowsPerLane];
-            row_phases(d, Dc, kk, tab, mq, lane, w, offs);
+            const QSFT_GLOBAL int8_t* __restrict__ Dc = gptr(d.D) + (size_t)c * P * n_pad;
+            double2 w[NR];
+            row_phases(d, Dc, kk, tab, mq, lane, w, offbuf, offs_staged);
             double rr = 0.0, ri = 0.0;
 #pragma unroll
-            for (int r = 0; r < kRowsPerLane; ++r) {
+            for (int r = 0; r < NR; ++r) {
                 const int p = lane + 64 * r;
                 if (p < P) {
                     rr += w[r].x * u[r].x + w[r].y * u[r].y;       // conj(w) * u
@@ -565,10 +705,11 @@ __global__ __launch_bounds__(kBinThreads) __attribute__((amdgpu_waves_per_eu(3))
             }
             rr = wave_sum(rr);
             ri = wave_sum(ri);
-            const double rho_x = __ddiv_rn(rr, (double)P), rho_y = __ddiv_rn(ri, (double)P);
+            rho_x = __ddiv_rn(rr, (double)P);
+            rho_y = __ddiv_rn(ri, (double)P);
             double qre = 0.0, qim = 0.0;
 #pragma unroll
-            for (int r = 0; r < kRowsPerLane; ++r) {
+            for (int r = 0; r < NR; ++r) {
                 const int p = lane + 64 * r;
                 if (p < P) {
                     const double sx = __dsub_rn(__dmul_rn(rho_x, w[r].x), __dmul_rn(rho_y, w[r].y));
@@ -583,34 +724,43 @@ __global__ __launch_bounds__(kBinThreads) __attribute__((amdgpu_waves_per_eu(3))
             double res = sqrt(qre + qim);
             res = res * res;
 
-            // --- bin match: M_c^T k mod q == digits of j (MSB first), one digit per lane
-            bool ok = true;
+            // --- bin match: the bin of k in group c, sum over digits dg (lane dg)
+            // of (<M_c^T row dg, k> mod q) q^(b - 1 - dg), equals j
+            int hd = 0;
             if (lane < b) {
-                const int dig = (j / digit_weight(q, b, lane)) % q;
-                const int8_t* mrow = mtab + ((size_t)c * b + lane) * n_pad;
-                ok = (int)mq(mtab == d.Mt ? dot_row(mrow, kk, nw) : dot_row_lds(mrow, kk, nw)) == dig;
+                const size_t mo = ((size_t)c * b + lane) * n_pad;
+                hd = (int)mq(mt_staged ? dot_row_lds(mtbuf + mo, kk, nw) : dot_row(gptr(d.Mt) + mo, kk, nw)) * wdig;
             }
-            const bool match = __all(ok);
-            const bool single = match && res <= thr;
-
-            if (lane == 0) d.mark[t] = it + 1;                      // non-zeroton: rescanned next pass
+            const bool match = __builtin_amdgcn_readfirstlane(row_sum_i(hd)) == j;   // b <= 16: lanes of row 0
+            single = match && res <= thr;
+            mval = single ? it + 1 : (it + 1) | kHeldMulti;        // non-zeroton: due next pass
             if (single) {
-                if (lane < nw) reinterpret_cast<uint32_t*>(kcur + (size_t)t * n_pad)[lane] = kk[lane];
+                const uint64_t hk = hash_k(kk, nw, lane);
                 if (lane == 0) {
-                    rcur[2 * (size_t)t] = rho_x;
-                    rcur[2 * (size_t)t + 1] = rho_y;
                     sb_t[wave][ns] = t;
-                    sb_h[wave][ns] = hash_words(kk, nw);
+                    sb_h[wave][ns] = hk;
                     sb_rho[wave][ns] = make_double2(rho_x, rho_y);
                 }
                 ++ns;
             }
-            wave_lds_sync();                                       // kk / cw are rewritten by the next bin
+        }
+        wait_vm<0>();                                              // the next bin's data has landed
+        __asm__ volatile("" ::: "memory");
+        // this bin's stores
+        if (pd.ent >= 0) store_col(d, t, u, lane);
+        if (mval && lane == 0) mark[t] = mval;
+        if (single) {
+            if (lane < nw) reinterpret_cast<QSFT_GLOBAL uint32_t*>(kcur + (size_t)t * n_pad)[lane] = kk[lane];
+            if (lane == 0) {
+                rcur[2 * (size_t)t] = rho_x;
+                rcur[2 * (size_t)t + 1] = rho_y;
+            }
         }
         if (!more) break;
         cur = nxt;
+        curent = nent;
         cslot ^= 1;
-        wave_lds_sync();                                           // this slot's LDS writes done before it is refilled
+        wave_lds_sync();                                           // kk / this slot are rewritten next
     }
 
     if (lane == 0) {
@@ -672,15 +822,17 @@ __global__ __launch_bounds__(kBinThreads) __attribute__((amdgpu_waves_per_eu(3))
     }
 }
 
-// Kernel B. Same bin partition as kernel A; the wave walks the singletons of
-// this pass among its 64 bins (sstamp == pass + 1). For each: the k must equal
-// its slot's key; the last detection of the key queues the ball on its bin in
-// every group (one target per group, C x b bin digits computed by as many
-// lanes at once; one atomicExch per target) and stamps the target for the
-// next pass, whose kernel A subtracts it from the column it owns.
+// Kernel B. Same bin partition as kernel A; lane L of a wave handles bin
+// 64 w + L if this pass found a singleton there (sstamp == pass + 1), all of
+// the wave's singletons at once: the detection's k (held in registers) must
+// equal its slot's key; the last detection of the key queues the ball on its
+// bin in every group (the C x b bin digits are dot products of the k with the
+// M^T rows; one atomicExch per target) and stamps the target for the next
+// pass, whose kernel A subtracts it from the column it owns.
+constexpr int kKeyChunks = QSFT_MAX_N / 16;                        // 16-byte chunks of a k row
+
 __global__ __launch_bounds__(kBinThreads) void k_peel_apply(const qsft_peel_desc* __restrict__ descs,
                                                             const int32_t* __restrict__ pre, int n_inst, int it) {
-    __shared__ __align__(16) uint32_t kws[kWaves][QSFT_MAX_N / 4];
     const int inst = find_inst(pre, n_inst, blockIdx.x);
     const qsft_peel_desc& d = descs[inst];
     uint32_t* __restrict__ ctr = d.ctr;
@@ -703,79 +855,76 @@ __global__ __launch_bounds__(kBinThreads) void k_peel_apply(const qsft_peel_desc
     }
     if (!active) return;
     const int q = d.q, b = d.b, C = d.C, Q = d.Q, n_pad = d.n_pad;
-    const int nw = n_pad >> 2;
+    const int nq = n_pad >> 4;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int CQ = C * Q;
     const int base = (local * kWaves + wave) * kBinsPerWave;
     if (base >= CQ) return;
-    const bool is_single = base + lane < CQ && d.sstamp[base + lane] == it + 1;
-    uint64_t todo = __ballot(is_single);
-    if (todo == 0) return;
-    uint32_t* kk = kws[wave];
+    const int t = base + lane;
+    if (!(t < CQ && d.sstamp[t] == it + 1)) return;                // no cross-lane work below
     const ModQ mq((uint32_t)q);
-    const bool all_digits = C * b <= 64;          // every group's bin digits in one step
     const int8_t* __restrict__ kcur = d.kbin + (size_t)(it & 1) * CQ * n_pad;
-
-    while (todo) {
-        const int t = base + __builtin_ctzll(todo);
-        todo &= todo - 1;
-        const int slot = d.slot_bin[t];
-        const uint64_t tm = ((uint64_t)(it + 1) << 32) | (uint32_t)t;
-        const uint32_t* ksrc = reinterpret_cast<const uint32_t*>(kcur + (size_t)t * n_pad);
-        if (lane < nw) kk[lane] = ksrc[lane];
-        wave_lds_sync();
-        // the detection's k must equal its slot's key
-        const uint64_t own = d.owner[slot];
-        uint32_t* kslot = reinterpret_cast<uint32_t*>(d.key + (size_t)slot * n_pad);
-        bool same = true;
-        if ((own >> 32) == (uint64_t)(it + 1)) {
-            if (own == tm) {
-                if (lane < nw) kslot[lane] = kk[lane];
-            } else if (lane < nw) {
-                same = reinterpret_cast<const uint32_t*>(kcur + (size_t)(uint32_t)own * n_pad)[lane] == kk[lane];
-            }
-        } else if (lane < nw) {
-            same = kslot[lane] == kk[lane];
-        }
-        if (!__all(same) && lane == 0) atomicOr(&ctr[QSFT_CTR_ERROR], (uint32_t)QSFT_DEVERR_TAG_COLLISION);
-        if (d.last[slot] == tm) {
-            // the last detection of this key in this pass queues it on every group
-            if (all_digits) {
-                int hd = 0;
-                if (lane < C * b) {
-                    const int l = lane / b, dg = lane - l * b;
-                    hd = (int)mq(dot_row(d.Mt + ((size_t)l * b + dg) * n_pad, kk, nw)) * digit_weight(q, b, dg);
-                }
-                // lane l < C sums the digits of group l (lanes l b .. l b + b - 1)
-                int jl = 0;
-                for (int dg = 0; dg < b; ++dg) {
-                    const int v = __shfl(hd, (lane < C ? lane : 0) * b + dg);
-                    jl += v;
-                }
-                if (lane < C) {
-                    const int target = lane * Q + jl;
-                    const int e = t * C + lane;
-                    d.pend_next[e] = atomicExch(&d.pend_head[target], e);
-                    d.mark[target] = it + 1;                        // the peeled bin is rescanned
-                }
-            } else {
-                for (int l = 0; l < C; ++l) {
-                    const int part = (lane < b) ? (int)mq(dot_row(d.Mt + ((size_t)l * b + lane) * n_pad, kk, nw)) *
-                                                      digit_weight(q, b, lane)
-                                                : 0;
-                    const int jl = wave_sum_i(part);
-                    if (lane == 0) {
-                        const int target = l * Q + jl;
-                        const int e = t * C + l;
-                        d.pend_next[e] = atomicExch(&d.pend_head[target], e);
-                        d.mark[target] = it + 1;
-                    }
-                }
+    const uint4* __restrict__ krow = reinterpret_cast<const uint4*>(kcur + (size_t)t * n_pad);
+    uint4 k[kKeyChunks];
+#pragma unroll
+    for (int w = 0; w < kKeyChunks; ++w) k[w] = (w < nq) ? krow[w] : make_uint4(0u, 0u, 0u, 0u);
+    const int slot = d.slot_bin[t];
+    const uint64_t tm = ((uint64_t)(it + 1) << 32) | (uint32_t)t;
+    const uint64_t own = d.owner[slot];
+    const uint64_t lst = d.last[slot];
+    // the detection's k must equal its slot's key
+    uint4* kslot = reinterpret_cast<uint4*>(d.key + (size_t)slot * n_pad);
+    bool same = true;
+    if (own == tm) {                                               // first detection of the key: it writes the key
+#pragma unroll
+        for (int w = 0; w < kKeyChunks; ++w)
+            if (w < nq) kslot[w] = k[w];
+    } else {
+        // a key first seen in this pass is compared with that detection's k
+        // (its key row may not be written yet), an older key with its key row
+        const uint4* ref = ((own >> 32) == (uint64_t)(it + 1))
+                               ? reinterpret_cast<const uint4*>(kcur + (size_t)(uint32_t)own * n_pad)
+                               : kslot;
+#pragma unroll
+        for (int w = 0; w < kKeyChunks; ++w) {
+            if (w < nq) {
+                const uint4 r = ref[w];
+                same = same && r.x == k[w].x && r.y == k[w].y && r.z == k[w].z && r.w == k[w].w;
             }
         }
-        wave_lds_sync();                                           // kk is rewritten by the next singleton
+    }
+    if (!same) atomicOr(&ctr[QSFT_CTR_ERROR], (uint32_t)QSFT_DEVERR_TAG_COLLISION);
+    if (lst != tm) return;
+    // the last detection of this key in this pass queues it on every group:
+    // bin of group l = sum_dg (<M_l^T row dg, k> mod q) q^(b - 1 - dg), MSB first
+    for (int l = 0; l < C; ++l) {
+        int jl = 0;
+        for (int dg = 0; dg < b; ++dg) {
+            const uint4* __restrict__ mrow = reinterpret_cast<const uint4*>(d.Mt + ((size_t)l * b + dg) * n_pad);
+            int acc = 0;
+#pragma unroll
+            for (int w = 0; w < kKeyChunks; ++w) {
+                if (w < nq) {
+                    const uint4 mv = mrow[w];
+                    acc = __builtin_amdgcn_sdot4((int)mv.x, (int)k[w].x, acc, false);
+                    acc = __builtin_amdgcn_sdot4((int)mv.y, (int)k[w].y, acc, false);
+                    acc = __builtin_amdgcn_sdot4((int)mv.z, (int)k[w].z, acc, false);
+                    acc = __builtin_amdgcn_sdot4((int)mv.w, (int)k[w].w, acc, false);
+                }
+            }
+            jl = jl * q + (int)mq((uint32_t)acc);
+        }
+        const int target = l * Q + jl;
+        const int e = t * C + l;
+        d.pend_next[e] = atomicExch(&d.pend_head[target], e);
+        d.mark[target] = it + 1;                                   // the peeled bin is rescanned
     }
 }
+
+// Output of one key: the running mean of its records in time order. Up to
+// kFinRecs records are gathered into registers by one walk of the key's list
+// and ordered there; keys with more records walk the list once per record.
+constexpr int kFinRecs = 8;
 
 __global__ __launch_bounds__(kSlotThreads) void k_peel_finalize(const qsft_peel_desc* __restrict__ descs,
                                                                 const int32_t* __restrict__ pre, int n_inst) {
@@ -783,23 +932,21 @@ __global__ __launch_bounds__(kSlotThreads) void k_peel_finalize(const qsft_peel_
     const qsft_peel_desc& d = descs[inst];
     const int H = 1 << d.hcap_log2;
     const int h = (blockIdx.x - pre[inst]) * kSlotThreads + threadIdx.x;
-    if (h >= H || d.tag[h] == 0) return;
-    const uint32_t m = d.count[h];
+    const int lane = threadIdx.x & 63;
+    const uint32_t m = (h < H && d.tag[h] != 0) ? d.count[h] : 0u;
+    const uint64_t live = __ballot(m != 0);
+    if (live == 0) return;                                         // the whole wave
+    // output rows: one counter atomic per wave
+    const int first = __builtin_ctzll(live);
+    uint32_t obase = 0;
+    if (lane == first) obase = atomicAdd(&d.ctr[QSFT_CTR_OUT], (uint32_t)__popcll(live));
+    obase = (uint32_t)__shfl((int)obase, first);
     if (m == 0) return;
+    const uint32_t o = obase + (uint32_t)__popcll(live & ((1ull << lane) - 1ull));
     const uint32_t L = d.ctr[QSFT_CTR_LOG];
     // running mean over the records in time order: g <- (g * c + v) / (c + 1)
-    uint64_t prev = 0;
     double gx = 0.0, gy = 0.0;
-    for (uint32_t cnt = 0; cnt < m; ++cnt) {
-        uint64_t best = ~0ull;
-        uint32_t bi = kNil;
-        uint32_t steps = 0;
-        for (uint32_t r = d.head[h]; r != kNil && steps <= L; r = d.rec_next[r], ++steps) {
-            const uint64_t tmr = d.rec_time[r];
-            if (tmr > prev && tmr < best) { best = tmr; bi = r; }
-        }
-        if (bi == kNil) break;
-        const double vx = d.rec_rho[2 * (size_t)bi], vy = d.rec_rho[2 * (size_t)bi + 1];
+    auto accumulate = [&](uint32_t cnt, double vx, double vy) {
         if (cnt == 0) {
             gx = vx;
             gy = vy;
@@ -808,9 +955,55 @@ __global__ __launch_bounds__(kSlotThreads) void k_peel_finalize(const qsft_peel_
             gx = __ddiv_rn(__dadd_rn(__dmul_rn(gx, cd), vx), cd + 1.0);
             gy = __ddiv_rn(__dadd_rn(__dmul_rn(gy, cd), vy), cd + 1.0);
         }
-        prev = best;
+    };
+    if (m <= (uint32_t)kFinRecs) {
+        uint64_t tr[kFinRecs];
+        double rx[kFinRecs], ry[kFinRecs];
+        uint32_t got = 0;
+        uint32_t r = d.head[h];
+#pragma unroll
+        for (int a = 0; a < kFinRecs; ++a) {
+            tr[a] = ~0ull;
+            rx[a] = ry[a] = 0.0;
+            if ((uint32_t)a < m && r != kNil) {
+                tr[a] = d.rec_time[r];
+                rx[a] = d.rec_rho[2 * (size_t)r];
+                ry[a] = d.rec_rho[2 * (size_t)r + 1];
+                r = d.rec_next[r];
+                ++got;
+            }
+        }
+        // times are distinct (one record per pass and bin): rank a = position of record a
+        uint64_t prev = 0;
+        for (uint32_t cnt = 0; cnt < got; ++cnt) {
+            uint64_t best = ~0ull;
+            double vx = 0.0, vy = 0.0;
+#pragma unroll
+            for (int a = 0; a < kFinRecs; ++a) {
+                if (tr[a] > prev && tr[a] < best) {
+                    best = tr[a];
+                    vx = rx[a];
+                    vy = ry[a];
+                }
+            }
+            accumulate(cnt, vx, vy);
+            prev = best;
+        }
+    } else {
+        uint64_t prev = 0;
+        for (uint32_t cnt = 0; cnt < m; ++cnt) {
+            uint64_t best = ~0ull;
+            uint32_t bi = kNil;
+            uint32_t steps = 0;
+            for (uint32_t r = d.head[h]; r != kNil && steps <= L; r = d.rec_next[r], ++steps) {
+                const uint64_t tmr = d.rec_time[r];
+                if (tmr > prev && tmr < best) { best = tmr; bi = r; }
+            }
+            if (bi == kNil) break;
+            accumulate(cnt, d.rec_rho[2 * (size_t)bi], d.rec_rho[2 * (size_t)bi + 1]);
+            prev = best;
+        }
     }
-    const uint32_t o = atomicAdd(&d.ctr[QSFT_CTR_OUT], 1u);
     if ((int)o >= d.out_cap) {
         atomicOr(&d.ctr[QSFT_CTR_ERROR], (uint32_t)QSFT_DEVERR_OUT_FULL);
         return;
@@ -828,21 +1021,38 @@ __global__ __launch_bounds__(kSlotThreads) void k_peel_finalize(const qsft_peel_
 // hdr[i] = count_i, hdr[n + i] = offset_i, hdr[2n] = total, hdr[2n + 1] = OR of error bits.
 __global__ __launch_bounds__(64) void k_pack_scan(const qsft_peel_desc* __restrict__ descs, int n_inst,
                                                   int64_t* __restrict__ hdr, int64_t cap_rows) {
-    if (threadIdx.x != 0) return;
-    int64_t off = 0;
+    const int lane = threadIdx.x;
+    long long carry = 0;
     uint32_t err = 0;
-    for (int i = 0; i < n_inst; ++i) {
-        const qsft_peel_desc& d = descs[i];
-        uint32_t m = d.ctr[QSFT_CTR_OUT];
-        if ((int)m > d.out_cap) m = (uint32_t)d.out_cap;
-        err |= d.ctr[QSFT_CTR_ERROR];
-        hdr[i] = m;
-        hdr[n_inst + i] = off;
-        off += m;
+    for (int i0 = 0; i0 < n_inst; i0 += 64) {                     // 64 instances per step, lane-parallel
+        const int i = i0 + lane;
+        long long m = 0;
+        if (i < n_inst) {
+            const qsft_peel_desc& d = descs[i];
+            uint32_t c = d.ctr[QSFT_CTR_OUT];
+            if ((int)c > d.out_cap) c = (uint32_t)d.out_cap;
+            err |= d.ctr[QSFT_CTR_ERROR];
+            m = c;
+        }
+        long long incl = m;                                        // inclusive scan over the lanes
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const long long v = __shfl_up(incl, off);
+            if (lane >= off) incl += v;
+        }
+        if (i < n_inst) {
+            hdr[i] = m;
+            hdr[n_inst + i] = carry + incl - m;
+        }
+        carry += __shfl(incl, 63);
     }
-    if (off > cap_rows) err |= QSFT_DEVERR_OUT_FULL;
-    hdr[2 * n_inst] = off;
-    hdr[2 * n_inst + 1] = err;
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) err |= (uint32_t)__shfl_xor((int)err, off);
+    if (lane == 0) {
+        if (carry > cap_rows) err |= QSFT_DEVERR_OUT_FULL;
+        hdr[2 * n_inst] = carry;
+        hdr[2 * n_inst + 1] = err;
+    }
 }
 
 // One block per instance copies its rows: k digits (n_pad bytes), value (2 doubles), order (u64).
@@ -1006,13 +1216,18 @@ extern "C" int qsft_peel_init(const qsft_peel_desc* d, const int32_t* pre_bins, 
 }
 
 extern "C" int qsft_peel_iteration(const qsft_peel_desc* d, const int32_t* pre_bins, int n_inst, int blocks_bins,
-                                   int it, void* stream) {
-    if (!d || !pre_bins || n_inst < 1 || blocks_bins < 1 || it < 0 || it >= 15) {
+                                   int max_p, int it, void* stream) {
+    if (!d || !pre_bins || n_inst < 1 || blocks_bins < 1 || it < 0 || it >= 15 || max_p < 1 ||
+        max_p > 64 * kRowsPerLane) {
         qsft_set_error("qsft_peel_iteration: bad arguments");
         return QSFT_ERR_ARG;
     }
     hipStream_t st = (hipStream_t)stream;
-    hipLaunchKernelGGL(k_peel_detect, dim3(blocks_bins), dim3(kBinThreads), 0, st, d, pre_bins, n_inst, it);
+    // rows per lane: 2 when every instance has P <= 128, else 4
+    if (max_p <= 128)
+        hipLaunchKernelGGL(k_peel_detect<2>, dim3(blocks_bins), dim3(kBinThreads), 0, st, d, pre_bins, n_inst, it);
+    else
+        hipLaunchKernelGGL(k_peel_detect<4>, dim3(blocks_bins), dim3(kBinThreads), 0, st, d, pre_bins, n_inst, it);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return qsft_hip_status(e, "qsft_peel_iteration(detect)");
     hipLaunchKernelGGL(k_peel_apply, dim3(blocks_bins), dim3(kBinThreads), 0, st, d, pre_bins, n_inst, it);
@@ -1030,10 +1245,11 @@ extern "C" int qsft_peel_finalize(const qsft_peel_desc* d, const int32_t* pre_sl
 }
 
 extern "C" int qsft_peel_run(const qsft_peel_desc* d, const int32_t* pre_bins, const int32_t* pre_slots, int n_inst,
-                             int blocks_bins, int blocks_slots, int max_iter, void* stream) {
+                             int blocks_bins, int blocks_slots, int max_p, int max_iter, void* stream) {
     if (max_iter < 1 || max_iter > 15) { qsft_set_error("qsft_peel_run: max_iter must be in [1, 15]"); return QSFT_ERR_ARG; }
     int rc = qsft_peel_init(d, pre_bins, pre_slots, n_inst, blocks_bins, blocks_slots, stream);
-    for (int it = 0; rc == QSFT_OK && it < max_iter; ++it) rc = qsft_peel_iteration(d, pre_bins, n_inst, blocks_bins, it, stream);
+    for (int it = 0; rc == QSFT_OK && it < max_iter; ++it)
+        rc = qsft_peel_iteration(d, pre_bins, n_inst, blocks_bins, max_p, it, stream);
     if (rc == QSFT_OK) rc = qsft_peel_finalize(d, pre_slots, n_inst, blocks_slots, stream);
     return rc;
 }

@@ -7,6 +7,23 @@
 
 namespace qsft {
 
+// Pointers read from a descriptor are generic; accesses through them compile to
+// flat_* instructions, which count on both the vector-memory and the LDS wait
+// counters (so an LDS wait also waits for them). Kernels mark global data with
+// gptr() so the accesses compile to global_* instructions.
+#define QSFT_GLOBAL __attribute__((address_space(1)))
+template <typename T> __device__ __forceinline__ QSFT_GLOBAL T* gptr(T* p) { return (QSFT_GLOBAL T*)p; }
+// Data that no thread writes while a kernel runs may be read through the
+// constant address space: wave-uniform reads then compile to scalar loads
+// (the LDS/scalar wait counter, not the vector-memory one).
+#define QSFT_CONST __attribute__((address_space(4)))
+template <typename T> __device__ __forceinline__ const QSFT_CONST T* cptr(const T* p) { return (const QSFT_CONST T*)p; }
+// Built-in vector types for such accesses (HIP's vector classes only copy in
+// the generic address space).
+typedef double gd2 __attribute__((ext_vector_type(2)));
+typedef int gi4 __attribute__((ext_vector_type(4)));
+typedef unsigned int gu4 __attribute__((ext_vector_type(4)));
+
 // Largest i with pre[i] <= blk: the instance that owns work block blk.
 __device__ __forceinline__ int find_inst(const int32_t* __restrict__ pre, int n_inst, int blk) {
     int lo = 0, hi = n_inst - 1;
@@ -90,10 +107,69 @@ __device__ __forceinline__ float4 normal4(uint64_t counter, uint64_t seed) {
                        rb * __builtin_amdgcn_cosf(u2b), rb * __builtin_amdgcn_sinf(u2b));
 }
 
-// xor-butterfly sum over the 64 lanes of a wave; every lane gets the same bits
+// Cross-lane moves for all-lane reductions, on the VALU (no LDS traffic).
+// Each step pairs every lane with one partner and both compute the same
+// a + b (commutative), so a reduction ends with the same bits in every lane.
+// DPP controls: quad_perm [1,0,3,2] (lane ^ 1), quad_perm [2,3,0,1]
+// (lane ^ 2), row_half_mirror (7 - i within 8 lanes), row_mirror (15 - i
+// within 16 lanes); then v_permlane16_swap pairs rows 0/1 and 2/3, and
+// v_permlane32_swap pairs the two halves of the wave. Every lane must be
+// active.
+template <int CTRL> __device__ __forceinline__ uint32_t dpp_u32(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, 0xF, 0xF, false);
+}
+template <int CTRL> __device__ __forceinline__ double dpp_f64(double v) {
+    const uint64_t b = __builtin_bit_cast(uint64_t, v);
+    const uint64_t r = (uint64_t)dpp_u32<CTRL>((uint32_t)b) | ((uint64_t)dpp_u32<CTRL>((uint32_t)(b >> 32)) << 32);
+    return __builtin_bit_cast(double, r);
+}
+// (own row pair member a, partner b) such that both lanes of the pair see the
+// values in the same order: .x from the lower row / half, .y from the upper
+__device__ __forceinline__ void swap16_u64(uint64_t v, uint64_t& lo_part, uint64_t& hi_part) {
+    const auto l = __builtin_amdgcn_permlane16_swap((uint32_t)v, (uint32_t)v, false, false);
+    const auto h = __builtin_amdgcn_permlane16_swap((uint32_t)(v >> 32), (uint32_t)(v >> 32), false, false);
+    lo_part = (uint64_t)l[0] | ((uint64_t)h[0] << 32);
+    hi_part = (uint64_t)l[1] | ((uint64_t)h[1] << 32);
+}
+__device__ __forceinline__ void swap32_u64(uint64_t v, uint64_t& lo_part, uint64_t& hi_part) {
+    const auto l = __builtin_amdgcn_permlane32_swap((uint32_t)v, (uint32_t)v, false, false);
+    const auto h = __builtin_amdgcn_permlane32_swap((uint32_t)(v >> 32), (uint32_t)(v >> 32), false, false);
+    lo_part = (uint64_t)l[0] | ((uint64_t)h[0] << 32);
+    hi_part = (uint64_t)l[1] | ((uint64_t)h[1] << 32);
+}
+
+// Sum over the 64 lanes of a wave; every lane gets the same bits.
 __device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
+    v += dpp_f64<0xB1>(v);
+    v += dpp_f64<0x4E>(v);
+    v += dpp_f64<0x141>(v);
+    v += dpp_f64<0x140>(v);
+    uint64_t a, b;
+    swap16_u64(__builtin_bit_cast(uint64_t, v), a, b);
+    v = __builtin_bit_cast(double, a) + __builtin_bit_cast(double, b);
+    swap32_u64(__builtin_bit_cast(uint64_t, v), a, b);
+    return __builtin_bit_cast(double, a) + __builtin_bit_cast(double, b);
+}
+
+// XOR over the 64 lanes of a wave.
+__device__ __forceinline__ uint64_t wave_xor64(uint64_t v) {
+    auto step = [](uint64_t x, uint64_t y) { return x ^ y; };
+    v = step(v, (uint64_t)dpp_u32<0xB1>((uint32_t)v) | ((uint64_t)dpp_u32<0xB1>((uint32_t)(v >> 32)) << 32));
+    v = step(v, (uint64_t)dpp_u32<0x4E>((uint32_t)v) | ((uint64_t)dpp_u32<0x4E>((uint32_t)(v >> 32)) << 32));
+    v = step(v, (uint64_t)dpp_u32<0x141>((uint32_t)v) | ((uint64_t)dpp_u32<0x141>((uint32_t)(v >> 32)) << 32));
+    v = step(v, (uint64_t)dpp_u32<0x140>((uint32_t)v) | ((uint64_t)dpp_u32<0x140>((uint32_t)(v >> 32)) << 32));
+    uint64_t a, b;
+    swap16_u64(v, a, b);
+    swap32_u64(a ^ b, a, b);
+    return a ^ b;
+}
+
+// Sum over the 16 lanes of each row (lanes 16 r .. 16 r + 15).
+__device__ __forceinline__ int row_sum_i(int v) {
+    v += (int)dpp_u32<0xB1>((uint32_t)v);
+    v += (int)dpp_u32<0x4E>((uint32_t)v);
+    v += (int)dpp_u32<0x141>((uint32_t)v);
+    v += (int)dpp_u32<0x140>((uint32_t)v);
     return v;
 }
 
@@ -104,10 +180,10 @@ constexpr int kColRowsPerLane = 4;
 // rows in order with explicit FMAs, an xor butterfly, then sqrt and square
 // (np.linalg.norm(col) ** 2). The assembler's energies, the peeler's screen
 // and its per-pass detection all call this, so they classify a column alike.
-__device__ __forceinline__ double bin_energy(const double2 (&u)[kColRowsPerLane]) {
+template <int N> __device__ __forceinline__ double bin_energy(const double2 (&u)[N]) {
     double sre = 0.0, sim = 0.0;
 #pragma unroll
-    for (int r = 0; r < kColRowsPerLane; ++r) {
+    for (int r = 0; r < N; ++r) {
         sre = __fma_rn(u[r].x, u[r].x, sre);
         sim = __fma_rn(u[r].y, u[r].y, sim);
     }
@@ -131,20 +207,39 @@ __device__ __forceinline__ uint32_t lds_addr(const void* p) {
 // right after. Every read of a destination slot follows an explicit wait_vm.
 // The global address is a wave-uniform base (SGPR pair) plus a 32-bit
 // per-lane byte offset.
-__device__ __forceinline__ void dma16(const void* sbase, uint32_t voff, uint32_t lds_base) {
+__device__ __forceinline__ uint64_t uniform_base(const void* sbase) {
     // the base is wave-uniform by contract; readfirstlane puts it in SGPRs. It
     // returns int: widen through uint32_t, or a low word >= 2^31 sign-extends
     // into the high word and the address is garbage.
     const uint64_t bv = (uint64_t)(uintptr_t)sbase;
     const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(bv >> 32));
     const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)bv);
-    const uint64_t bs = ((uint64_t)hi << 32) | (uint64_t)lo;
+    return ((uint64_t)hi << 32) | (uint64_t)lo;
+}
+
+__device__ __forceinline__ void dma16(const void* sbase, uint32_t voff, uint32_t lds_base) {
+    const uint64_t bs = uniform_base(sbase);
     uint32_t saved;
     asm volatile(
         "s_mov_b32 %0, m0\n\t"
         "s_mov_b32 m0, %3\n\t"
         "s_nop 0\n\t"
         "global_load_lds_dwordx4 %1, %2\n\t"
+        "s_mov_b32 m0, %0"
+        : "=&s"(saved)
+        : "v"(voff), "s"(bs), "s"(lds_base)
+        : "memory");
+}
+
+// The same for 4 bytes, issued by lane 0 alone, which writes lds_base.
+__device__ __forceinline__ void dma4(const void* sbase, uint32_t voff, uint32_t lds_base) {
+    const uint64_t bs = uniform_base(sbase);
+    uint32_t saved;
+    asm volatile(
+        "s_mov_b32 %0, m0\n\t"
+        "s_mov_b32 m0, %3\n\t"
+        "s_nop 0\n\t"
+        "global_load_lds_dword %1, %2\n\t"
         "s_mov_b32 m0, %0"
         : "=&s"(saved)
         : "v"(voff), "s"(bs), "s"(lds_base)

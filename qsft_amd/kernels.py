@@ -440,6 +440,7 @@ class PeelBatch:
         self.buf, self.dptr, (self.pre_b, self.pre_s), (self.tot_b, self.tot_s) = pack(
             self.descs, [self.bins, self.slots], dev)
         self.n_pad_max = max(d.n_pad for d in self.descs)
+        self.max_p = max(d.P for d in self.descs)
         self.row_stride = self.n_pad_max + 24
         self.cap_rows = sum(1 << d.hcap_log2 for d in self.descs) if cap_rows is None else int(cap_rows)
         self.hdr_bytes = lib.qsft_peel_pack_header_bytes(len(self.descs))
@@ -455,12 +456,13 @@ class PeelBatch:
         n = len(self.descs)
         d, pb, ps = _vp(self.dptr), _vp(self.pre_b), _vp(self.pre_s)
         if not host_loop:
-            L.check(lib.qsft_peel_run(d, pb, ps, n, self.tot_b, self.tot_s, self.max_iter, st), "qsft_peel_run")
+            L.check(lib.qsft_peel_run(d, pb, ps, n, self.tot_b, self.tot_s, self.max_p, self.max_iter, st),
+                    "qsft_peel_run")
         else:
             # host-driven loop: one sync per pass to read the stop flag (the plain path)
             L.check(lib.qsft_peel_init(d, pb, ps, n, self.tot_b, self.tot_s, st), "qsft_peel_init")
             for it in range(self.max_iter):
-                L.check(lib.qsft_peel_iteration(d, pb, n, self.tot_b, it, st), "qsft_peel_iteration")
+                L.check(lib.qsft_peel_iteration(d, pb, n, self.tot_b, self.max_p, it, st), "qsft_peel_iteration")
                 if it + 1 >= self.max_iter:
                     break
                 if not any(self._ctr_word(i, (it + 1) * L.QSFT_CTR_ITER_STRIDE + 3) for i in range(n)):
