@@ -205,10 +205,15 @@ typedef struct qsft_assemble_desc {
 } qsft_assemble_desc;
 
 int qsft_assemble_blocks(const qsft_assemble_desc* host_desc);
+/* Launch word bits of one instance; the launch takes the AND over its
+ * instances. QSFT_ASM_F32_TILE: FP32 source and no noise array, so the
+ * bin-major tile may be held in FP32 (half the LDS per workgroup). */
+#define QSFT_ASM_F32_TILE 1
+int qsft_assemble_word(const qsft_assemble_desc* host_desc);
 /* seed_offset is added to every descriptor's Philox seed, so one uploaded
  * batch can draw fresh noise on every launch. */
 int qsft_assemble(const qsft_assemble_desc* dev_descs, const int32_t* dev_pre, int n_inst,
-                  int total_blocks, uint64_t seed_offset, void* stream);
+                  int total_blocks, uint64_t seed_offset, int word, void* stream);
 
 /* ------------------------------------------------------------------------
  * Stage (d): singleton detection + peeling, device resident.

@@ -98,7 +98,8 @@ SIGNATURES = {
                                      vp]),
     "qsft_dft4_rows": (ctypes.c_int, [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp]),
     "qsft_assemble_blocks": (ctypes.c_int, [ctypes.POINTER(AssembleDesc)]),
-    "qsft_assemble": (ctypes.c_int, [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_uint64, vp]),
+    "qsft_assemble_word": (ctypes.c_int, [ctypes.POINTER(AssembleDesc)]),
+    "qsft_assemble": (ctypes.c_int, [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.c_int, vp]),
     "qsft_peel_workspace_bytes": (ctypes.c_size_t, [ctypes.c_int] * 7),
     "qsft_peel_bind": (ctypes.c_int, [ctypes.POINTER(PeelDesc), ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                       ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,

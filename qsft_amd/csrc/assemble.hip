@@ -47,10 +47,17 @@ __device__ __forceinline__ double2 add_array_noise(const qsft_assemble_desc& d, 
     return v;
 }
 
+__device__ __forceinline__ double2 to_d2(double2 v) { return v; }
+__device__ __forceinline__ double2 to_d2(float2 v) { return make_double2(v.x, v.y); }
+
+// TT: element type of the LDS tile. float2 (half the LDS, twice the
+// workgroups per CU) when every instance of the launch reads an FP32 source
+// without a noise array (the host passes that fact); double2 otherwise.
+template <typename TT>
 __global__ __launch_bounds__(kAsmThreads) void k_assemble(const qsft_assemble_desc* __restrict__ descs,
                                                           const int32_t* __restrict__ pre, int n_inst,
                                                           uint64_t seed_offset) {
-    __shared__ double2 tile[kBmRows][kBmBins + 1];
+    __shared__ TT tile[kBmRows][kBmBins + 1];
     __shared__ int64_t srow[kBmRows];
     const int inst = find_inst(pre, n_inst, blockIdx.x);
     const qsft_assemble_desc& d = descs[inst];
@@ -96,28 +103,45 @@ __global__ __launch_bounds__(kAsmThreads) void k_assemble(const qsft_assemble_de
     // read: 16 lanes cover one row's 16-bin segment; all loads of a thread are
     // issued before any is stored
     constexpr int kReads = kBmRows * kBmBins / kAsmThreads;       // 8
-    double2 v[kReads];
+    if constexpr (sizeof(TT) == 8) {                               // FP32 source, no noise array
+        float2 v[kReads];
 #pragma unroll
-    for (int k = 0; k < kReads; ++k) {
-        const int e = threadIdx.x + k * kAsmThreads;
-        const int pr = e >> 4, jj = e & 15;
-        v[k] = make_double2(0.0, 0.0);
-        if (pr < Pc && jj < Jc) {
-            const int64_t si = srow[pr] + j0 + jj;
-            if (d.src_prec == QSFT_F32) {
-                const float2 f = reinterpret_cast<const float2*>(d.src)[si];
-                v[k] = make_double2(f.x, f.y);
-            } else {
-                v[k] = reinterpret_cast<const double2*>(d.src)[si];
+        for (int k = 0; k < kReads; ++k) {
+            const int e = threadIdx.x + k * kAsmThreads;
+            const int pr = e >> 4, jj = e & 15;
+            v[k] = make_float2(0.0f, 0.0f);
+            if (pr < Pc && jj < Jc) v[k] = reinterpret_cast<const float2*>(d.src)[srow[pr] + j0 + jj];
+        }
+#pragma unroll
+        for (int k = 0; k < kReads; ++k) {
+            const int e = threadIdx.x + k * kAsmThreads;
+            const int pr = e >> 4, jj = e & 15;
+            if (pr < Pc && jj < Jc) tile[pr][jj] = v[k];
+        }
+    } else {
+        double2 v[kReads];
+#pragma unroll
+        for (int k = 0; k < kReads; ++k) {
+            const int e = threadIdx.x + k * kAsmThreads;
+            const int pr = e >> 4, jj = e & 15;
+            v[k] = make_double2(0.0, 0.0);
+            if (pr < Pc && jj < Jc) {
+                const int64_t si = srow[pr] + j0 + jj;
+                if (d.src_prec == QSFT_F32) {
+                    const float2 f = reinterpret_cast<const float2*>(d.src)[si];
+                    v[k] = make_double2(f.x, f.y);
+                } else {
+                    v[k] = reinterpret_cast<const double2*>(d.src)[si];
+                }
             }
         }
-    }
 #pragma unroll
-    for (int k = 0; k < kReads; ++k) {
-        const int e = threadIdx.x + k * kAsmThreads;
-        const int pr = e >> 4, jj = e & 15;
-        if (pr < Pc && jj < Jc)
-            tile[pr][jj] = add_array_noise(d, v[k], ((int64_t)c * rows + p0 + pr) * Q + j0 + jj);
+        for (int k = 0; k < kReads; ++k) {
+            const int e = threadIdx.x + k * kAsmThreads;
+            const int pr = e >> 4, jj = e & 15;
+            if (pr < Pc && jj < Jc)
+                tile[pr][jj] = add_array_noise(d, v[k], ((int64_t)c * rows + p0 + pr) * Q + j0 + jj);
+        }
     }
     __syncthreads();
     // write: wave w takes bin pairs (2h, 2h + 1), h = w, w + 4, ...; lane L holds
@@ -142,8 +166,8 @@ __global__ __launch_bounds__(kAsmThreads) void k_assemble(const qsft_assemble_de
             const int pr = lane + 64 * r;
             if (pr < Pc) {
                 const int p = p0 + pr;
-                double2 v0 = tile[pr][jj];
-                double2 v1 = two ? tile[pr][jj + 1] : make_double2(0.0, 0.0);
+                double2 v0 = to_d2(tile[pr][jj]);
+                double2 v1 = two ? to_d2(tile[pr][jj + 1]) : make_double2(0.0, 0.0);
                 if (philox) {
                     const float4 z = normal4(noise_counter(c, p, rows, Q, j), seed);
                     v0.x = v0.x + comp_sd * (double)z.x;
@@ -191,11 +215,23 @@ extern "C" int qsft_assemble_blocks(const qsft_assemble_desc* h) {
     return blocks > 0x7FFFFFFF ? -1 : (int)blocks;
 }
 
+extern "C" int qsft_assemble_word(const qsft_assemble_desc* h) {
+    if (!h) return 0;
+    return (h->src_prec == QSFT_F32 && h->noise_mode != QSFT_NOISE_ARRAY) ? QSFT_ASM_F32_TILE : 0;
+}
+
 extern "C" int qsft_assemble(const qsft_assemble_desc* d, const int32_t* pre, int n_inst, int blocks,
-                             uint64_t seed_offset, void* stream) {
-    if (!d || !pre || n_inst < 1 || blocks < 0) { qsft_set_error("qsft_assemble: bad arguments"); return QSFT_ERR_ARG; }
+                             uint64_t seed_offset, int word, void* stream) {
+    if (!d || !pre || n_inst < 1 || blocks < 0 || (word & ~QSFT_ASM_F32_TILE)) {
+        qsft_set_error("qsft_assemble: bad arguments");
+        return QSFT_ERR_ARG;
+    }
     if (blocks == 0) return QSFT_OK;
-    hipLaunchKernelGGL(k_assemble, dim3(blocks), dim3(kAsmThreads), 0, (hipStream_t)stream, d, pre, n_inst,
-                       seed_offset);
+    if (word & QSFT_ASM_F32_TILE)
+        hipLaunchKernelGGL(k_assemble<float2>, dim3(blocks), dim3(kAsmThreads), 0, (hipStream_t)stream, d, pre,
+                           n_inst, seed_offset);
+    else
+        hipLaunchKernelGGL(k_assemble<double2>, dim3(blocks), dim3(kAsmThreads), 0, (hipStream_t)stream, d, pre,
+                           n_inst, seed_offset);
     return qsft_hip_status(hipGetLastError(), "qsft_assemble");
 }

@@ -104,7 +104,7 @@ class Batch:
     zeroton screen does not read U again."""
 
     def __init__(self, specs, precision="fp32", max_iter=15, device=None, eval_mode=None, fused_screen=True,
-                 host_rows=True):
+                 host_rows=True, tail_stream=True):
         self.specs = list(specs)
         self.precision = precision
         if eval_mode is None:
@@ -148,6 +148,12 @@ class Batch:
         # in pinned host memory that the pack kernels write directly
         cap = sum(4 * sp.S + 1024 for sp in self.specs) if host_rows else None
         self.peel = K.PeelBatch(pl, max_iter=max_iter, host_rows=host_rows, cap_rows=cap)
+        # the peeling tail (last queued balls, output table, pack into host
+        # memory) runs on a second stream, overlapping the next batch's work;
+        # the next launch of this batch waits for it
+        self.tail = torch.cuda.Stream(device=dev) if (host_rows and tail_stream) else None
+        self.tail_event = torch.cuda.Event() if self.tail is not None else None
+        self.done = None
 
     def launch(self, seed_offset=0, host_loop=False, events=None, marks=None):
         """Enqueue the whole pipeline on the current stream. With `events`
@@ -161,6 +167,8 @@ class Batch:
                 e = torch.cuda.Event(enable_timing=True)
                 e.record()
                 events.setdefault(name, []).append(e)
+        if self.done is not None and self.tail is not None:
+            torch.cuda.current_stream().wait_event(self.done)      # the previous tail still reads the tables
         mark("start")
         if (events is not None or marks is not None) and hasattr(self.eval, "launch_split"):
             self.eval.launch_split(lambda: mark("eval_prep"))
@@ -171,16 +179,21 @@ class Batch:
         mark("dft")
         self.asm.launch(seed_offset)
         mark("assemble")
-        self.peel.run(host_loop=host_loop)
-        mark("peel")
+        self.peel.run(host_loop=host_loop, tail_stream=self.tail, tail_event=self.tail_event)
         # one completion event per batch, re-recorded every launch (a copy that
         # waits on it captured the previous record when it was enqueued)
-        if getattr(self, "done", None) is None:
+        if self.done is None:
             self.done = torch.cuda.Event()
-        self.done.record()
+        if self.tail is not None:
+            with torch.cuda.stream(self.tail):
+                mark("peel")
+                self.done.record()
+        else:
+            mark("peel")
+            self.done.record()
 
     def results(self, with_info=False):
-        return self.peel.results(with_info=with_info)
+        return self.peel.results(with_info=with_info, packed=self.fetch())
 
     def fetch(self, copy_stream=None):
         """Packed rows of the last launch once it is done: (counts, offsets, rows).

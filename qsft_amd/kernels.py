@@ -353,12 +353,16 @@ class Assemble:
                 raise ValueError("bad assemble shape (energies need bin-major U with R*p1 <= 128)")
             descs.append(d)
             blocks.append(nb)
+        self.word = -1
+        for d in descs:                                            # launch word: AND over the instances
+            self.word &= lib.qsft_assemble_word(ctypes.byref(d))
         self.buf, self.dptr, (self.pre,), (self.tot,) = pack(descs, [blocks], items[0]["U"].device)
         self.n = len(descs)
 
     def launch(self, seed_offset=0):
         L.check(lib.qsft_assemble(_vp(self.dptr), _vp(self.pre), self.n, self.tot,
-                                  int(seed_offset) & 0xFFFFFFFFFFFFFFFF, stream_handle()), "qsft_assemble")
+                                  int(seed_offset) & 0xFFFFFFFFFFFFFFFF, self.word, stream_handle()),
+                "qsft_assemble")
 
 
 def assemble(items):
@@ -451,22 +455,30 @@ class PeelBatch:
             self.packed = torch.empty(size, dtype=torch.uint8, device=dev)
             self.hdr_host = torch.empty(self.hdr_bytes, dtype=torch.uint8).pin_memory()
 
-    def run(self, host_loop=False):
+    def run(self, host_loop=False, tail_stream=None, tail_event=None):
+        """The peeling passes on the current stream, then the tail (last
+        pass's queued balls, output table, pack). With `tail_stream` the tail
+        runs there, after `tail_event` (recorded here on the current stream),
+        so the current stream can go on with other work meanwhile."""
         st = stream_handle()
         n = len(self.descs)
         d, pb, ps = _vp(self.dptr), _vp(self.pre_b), _vp(self.pre_s)
-        if not host_loop:
+        if not host_loop and tail_stream is None:
             L.check(lib.qsft_peel_run(d, pb, ps, n, self.tot_b, self.tot_s, self.max_p, self.max_iter, st),
                     "qsft_peel_run")
         else:
-            # host-driven loop: one sync per pass to read the stop flag (the plain path)
             L.check(lib.qsft_peel_init(d, pb, ps, n, self.tot_b, self.tot_s, st), "qsft_peel_init")
             for it in range(self.max_iter):
                 L.check(lib.qsft_peel_iteration(d, pb, n, self.tot_b, self.max_p, it, st), "qsft_peel_iteration")
-                if it + 1 >= self.max_iter:
-                    break
+                if not host_loop or it + 1 >= self.max_iter:
+                    continue
+                # host-driven loop: one sync per pass to read the stop flag (the plain path)
                 if not any(self._ctr_word(i, (it + 1) * L.QSFT_CTR_ITER_STRIDE + 3) for i in range(n)):
                     break
+            if tail_stream is not None:
+                tail_event.record()
+                tail_stream.wait_event(tail_event)
+                st = stream_handle(tail_stream)
             L.check(lib.qsft_peel_finalize(d, ps, n, self.tot_s, st), "qsft_peel_finalize")
         L.check(lib.qsft_peel_pack(d, n, _vp(self.packed.data_ptr()), self.cap_rows, self.row_stride, st),
                 "qsft_peel_pack")

@@ -256,6 +256,33 @@ def test_assemble_philox_noise_statistics():
     assert torch.equal(U, U2)                      # counter-based: same seed, same draws
 
 
+@pytest.mark.parametrize("R,p1,Q", [(2, 51, 4096), (1, 100, 33)])
+def test_assemble_f32_tile_matches_f64_tile(R, p1, Q):
+    """An FP32 source goes through the FP32 LDS tile; forcing the FP64 tile
+    gives the same U and energies bit for bit (the FP32 -> FP64 promotion is
+    exact wherever it happens)."""
+    print(f"assemble tiles R={R} p1={p1} Q={Q}", flush=True)
+    dev = dv.current_device()
+    C = 3
+    rng = np.random.default_rng(8)
+    src = torch.from_numpy(rng.normal(size=(C, R * p1, Q)) + 1j * rng.normal(size=(C, R * p1, Q)))
+    src = src.to(torch.complex64).to(dev)
+    out = []
+    for force64 in (False, True):
+        U = torch.empty((C, Q, R * p1), dtype=torch.complex128, device=dev)
+        e = torch.empty((C, Q), dtype=torch.float64, device=dev)
+        a = K.Assemble([dict(Q=Q, C=C, R=R, p1=p1, src=src, src_rows=R * p1, gsel=[2, 0, 1], rsel=list(range(R)),
+                             sigma=0.5, seed=7, U=U, energy=e, bin_major=True)])
+        assert a.word == 1                           # QSFT_ASM_F32_TILE
+        if force64:
+            a.word = 0
+        a.launch()
+        torch.cuda.synchronize()
+        out.append((U, e))
+    assert torch.equal(out[0][0], out[1][0])
+    assert torch.equal(out[0][1], out[1][1])
+
+
 @pytest.mark.parametrize("R,p1,Q", [(2, 51, 4096), (3, 43, 27), (1, 129, 20)])
 def test_assemble_philox_layout_independent(R, p1, Q):
     """The bin-major store draws exactly the row-major noise (the noise is a
