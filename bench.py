@@ -51,6 +51,9 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-workers", type=int, default=16)
     ap.add_argument("--quiet", action="store_true")
+    ap.add_argument("--streams", choices=("shared", "per-batch"), default="shared",
+                    help="shared: both resident batches on one stream (the peeling tail on a second); "
+                         "per-batch: each batch on its own stream, so their kernels interleave")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (a known-size reference for "
                          "hardware byte counters)")
@@ -128,7 +131,7 @@ def main():
     spec_sets = [[E.random_spec(q, n, S, b, C, R, snr, rng=rng) for _ in range(args.batch)] for _ in range(2)]
     # two resident batches alternate: while the host reads back one (its pack
     # kernels write the rows straight into pinned host memory), the GPU runs the other
-    engines = [E.Batch(sp, precision="fp32") for sp in spec_sets]
+    engines = [E.Batch(sp, precision="fp32", own_stream=args.streams == "per-batch") for sp in spec_sets]
     eng, specs = engines[0], spec_sets[0]
 
     log("warmup", args.quiet)
@@ -250,7 +253,7 @@ def main():
         "vs_baseline": None,
         "dtype": "fp32 eval/dft, fp64 peel",
         "data": "synthetic",
-        "config": {"workload": "q=4 n=50 S=1000 b=6 C=3 nso R=2 10dB", "batch": args.batch,
+        "config": {"workload": "q=4 n=50 S=1000 b=6 C=3 nso R=2 10dB", "batch": args.batch, "streams": args.streams,
                    "eval": eng.eval_mode,
                    "parallelism": "batched instances, 1 GPU"},
         "latency_ms": {"p50": 1e3 * float(np.percentile(lat, 50)), "p99": 1e3 * float(np.percentile(lat, 99)),

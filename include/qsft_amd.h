@@ -264,6 +264,7 @@ typedef struct qsft_peel_desc {
     uint64_t* last;         /* [H] last (iteration, bin) of the key                         */
     uint32_t* head;         /* [H] head of the key's record list                            */
     uint32_t* count;        /* [H] records of the key                                       */
+    uint32_t* live;         /* [H] claimed slots, in claim order (QSFT_CTR_LIVE of them)    */
     int8_t* key;            /* [H][n_pad] digits of the key, written by its first record    */
     uint64_t* rec_time;     /* [L] (iteration + 1) << 32 | bin                              */
     double* rec_rho;        /* [L][2]                                                       */
@@ -292,6 +293,7 @@ typedef struct qsft_peel_desc {
 #define QSFT_CTR_ERROR 130      /* QSFT_DEVERR_* bits                                  */
 #define QSFT_CTR_ITERS 131      /* iterations that ran                                 */
 #define QSFT_CTR_PEELED 132     /* balls peeled (saturating)                           */
+#define QSFT_CTR_LIVE 133       /* hash slots claimed (keys)                           */
 #define QSFT_CTR_WORDS 136
 
 size_t qsft_peel_workspace_bytes(int q, int n, int b, int C, int P, int max_iter, int out_cap);
@@ -316,8 +318,12 @@ int qsft_peel_init(const qsft_peel_desc* dev_descs, const int32_t* dev_pre_bins,
                    int n_inst, int blocks_bins, int blocks_slots, void* stream);
 int qsft_peel_iteration(const qsft_peel_desc* dev_descs, const int32_t* dev_pre_bins, int n_inst,
                         int blocks_bins, int max_p, int iteration, void* stream);
+/* word: QSFT_FIN_FLUSH first subtracts the balls the last pass queued, so U
+ * ends as the reference's loop leaves it; without it U keeps them queued
+ * (the recovered rows are the same either way). */
+#define QSFT_FIN_FLUSH 1
 int qsft_peel_finalize(const qsft_peel_desc* dev_descs, const int32_t* dev_pre_slots, int n_inst,
-                       int blocks_slots, void* stream);
+                       int blocks_slots, int word, void* stream);
 
 /* Gather every instance's recovered rows into one buffer `dst` (device or
  * host-mapped) so a batch comes back in one copy: an int64 header of

@@ -24,7 +24,8 @@ QSFT_MAX_Q, QSFT_MAX_N, QSFT_MAX_B, QSFT_MAX_C, QSFT_MAX_R = 64, 256, 16, 16, 16
 QSFT_DFT_MAX_PASSES = 4
 QSFT_PEEL_SHARED_D, QSFT_PEEL_UNIT_D, QSFT_PEEL_UNIT_D_NEG = 1, 2, 4
 QSFT_CTR_ITER_STRIDE = 8
-QSFT_CTR_LOG, QSFT_CTR_OUT, QSFT_CTR_ERROR, QSFT_CTR_ITERS, QSFT_CTR_PEELED = 128, 129, 130, 131, 132
+QSFT_CTR_LOG, QSFT_CTR_OUT, QSFT_CTR_ERROR, QSFT_CTR_ITERS, QSFT_CTR_PEELED, QSFT_CTR_LIVE = 128, 129, 130, 131, 132, 133
+QSFT_FIN_FLUSH = 1
 QSFT_CTR_WORDS = 136
 
 i32, i64, u64, f64, vp = ctypes.c_int32, ctypes.c_int64, ctypes.c_uint64, ctypes.c_double, ctypes.c_void_p
@@ -64,7 +65,7 @@ class PeelDesc(ctypes.Structure):
                 ("kbin", vp), ("rho_bin", vp), ("slot_bin", vp), ("mark", vp), ("sstamp", vp),
                 ("pend_head", vp), ("pend_next", vp),
                 ("tag", vp), ("owner", vp),
-                ("last", vp), ("head", vp), ("count", vp), ("key", vp), ("rec_time", vp),
+                ("last", vp), ("head", vp), ("count", vp), ("live", vp), ("key", vp), ("rec_time", vp),
                 ("rec_rho", vp), ("rec_next", vp), ("ctr", vp),
                 ("out_k", vp), ("out_val", vp), ("out_order", vp)]
 
@@ -111,7 +112,7 @@ SIGNATURES = {
                                      vp]),
     "qsft_peel_init": (ctypes.c_int, [vp, vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp]),
     "qsft_peel_iteration": (ctypes.c_int, [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp]),
-    "qsft_peel_finalize": (ctypes.c_int, [vp, vp, ctypes.c_int, ctypes.c_int, vp]),
+    "qsft_peel_finalize": (ctypes.c_int, [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp]),
     "qsft_peel_pack_header_bytes": (ctypes.c_size_t, [ctypes.c_int]),
     "qsft_peel_pack": (ctypes.c_int, [vp, ctypes.c_int, vp, ctypes.c_int64, ctypes.c_int, vp]),
 }

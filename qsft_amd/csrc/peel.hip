@@ -553,11 +553,23 @@ __global__ __launch_bounds__(kBinThreads) __attribute__((amdgpu_waves_per_eu(3))
     const int nw = n_pad >> 2;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int CQ = C * Q;
+    QSFT_GLOBAL int32_t* __restrict__ mark = gptr(d.mark);
+    uint32_t* kk = kws[wave];
+    int8_t* kb8 = reinterpret_cast<int8_t*>(kk);
+    const ModQ mq((uint32_t)q);
+    const double thr = d.cutoff * (double)P;
+    const double2* __restrict__ Ub = reinterpret_cast<const double2*>(d.U);
+    QSFT_GLOBAL int8_t* __restrict__ kcur = gptr(d.kbin) + (size_t)(it & 1) * CQ * n_pad;   // this pass's k and rho
+    QSFT_GLOBAL double* __restrict__ rcur = gptr(d.rho_bin) + (size_t)(it & 1) * CQ * 2;
+    int wdig = 0;                                                  // q^(b - 1 - lane): weight of bin digit `lane`
+    if (lane < b) {
+        wdig = 1;
+        for (int e = lane + 1; e < b; ++e) wdig *= q;
+    }
     const int base = ((blockIdx.x - pre[inst]) * kWaves + wave) * kBinsPerWave;
     // The wave's bins due this pass. A held multiton (a multiton last pass that
     // no ball has been queued on since) keeps its column, so it is a multiton
     // again: it is counted and re-stamped without being read.
-    QSFT_GLOBAL int32_t* __restrict__ mark = gptr(d.mark);
     const int mk = (base + lane < CQ) ? mark[base + lane] : -1;
     const bool due = (mk & ~kHeldMulti) == it;
     const bool held = due && (mk & kHeldMulti) != 0;
@@ -577,21 +589,8 @@ __global__ __launch_bounds__(kBinThreads) __attribute__((amdgpu_waves_per_eu(3))
         return;
     }
     const uint32_t scanned = (uint32_t)__popcll(todo);
-
-    uint32_t* kk = kws[wave];
-    int8_t* kb8 = reinterpret_cast<int8_t*>(kk);
-    const ModQ mq((uint32_t)q);
-    const double thr = d.cutoff * (double)P;
-    const double2* __restrict__ Ub = reinterpret_cast<const double2*>(d.U);
-    QSFT_GLOBAL int8_t* __restrict__ kcur = gptr(d.kbin) + (size_t)(it & 1) * CQ * n_pad;   // this pass's k and rho
-    QSFT_GLOBAL double* __restrict__ rcur = gptr(d.rho_bin) + (size_t)(it & 1) * CQ * 2;
     int nz = (int)nheld, ns = 0;                                   // wave-uniform
     int c = base / Q;                                              // group of the current bin
-    int wdig = 0;                                                  // q^(b - 1 - lane): weight of bin digit `lane`
-    if (lane < b) {
-        wdig = 1;
-        for (int e = lane + 1; e < b; ++e) wdig *= q;
-    }
 
     // queue heads of the wave's bins (balls the previous pass peeled into them)
     const int qhead = (it > 0 && due && !held) ? gptr(d.pend_head)[base + lane] : -1;
@@ -773,6 +772,7 @@ __global__ __launch_bounds__(kBinThreads) __attribute__((amdgpu_waves_per_eu(3))
 
     // --- hash registration, one lane per singleton of this wave
     int slot = -1;
+    bool claimed = false;                                      // this lane's k took a new slot
     bool newball = false;
     int t = 0;
     uint64_t tm = 0;
@@ -786,7 +786,11 @@ __global__ __launch_bounds__(kBinThreads) __attribute__((amdgpu_waves_per_eu(3))
         uint32_t pos = (uint32_t)(hv >> 32) & mask;
         for (uint32_t probe = 0; probe <= mask; ++probe) {
             const unsigned long long old = atomicCAS((unsigned long long*)&d.tag[pos], 0ull, (unsigned long long)tg);
-            if (old == 0ull || old == tg) { slot = (int)pos; break; }
+            if (old == 0ull || old == tg) {
+                slot = (int)pos;
+                claimed = old == 0ull;
+                break;
+            }
             pos = (pos + 1) & mask;
         }
         if (slot < 0) {
@@ -802,12 +806,16 @@ __global__ __launch_bounds__(kBinThreads) __attribute__((amdgpu_waves_per_eu(3))
     }
     const uint64_t nb_mask = __ballot(newball);
     const uint64_t ok_mask = __ballot(slot >= 0);
-    uint32_t rbase = 0;
+    const uint64_t cl_mask = __ballot(claimed);
+    uint32_t rbase = 0, lbase = 0;
     if (lane == 0) {
         if (nb_mask) atomicAdd(&ctr[it * QSFT_CTR_ITER_STRIDE + 2], (uint32_t)__popcll(nb_mask));
         if (ok_mask) rbase = atomicAdd(&ctr[QSFT_CTR_LOG], (uint32_t)__popcll(ok_mask));
+        if (cl_mask) lbase = atomicAdd(&ctr[QSFT_CTR_LIVE], (uint32_t)__popcll(cl_mask));
     }
     rbase = __shfl(rbase, 0);
+    lbase = __shfl(lbase, 0);
+    if (claimed) d.live[lbase + (uint32_t)__popcll(cl_mask & ((1ull << lane) - 1ull))] = (uint32_t)slot;
     if (slot >= 0) {
         const uint32_t rix = rbase + (uint32_t)__popcll(ok_mask & ((1ull << lane) - 1ull));
         if ((uint64_t)rix >= (uint64_t)d.max_iter * (uint64_t)CQ) {   // cannot happen: one record per bin per pass
@@ -822,46 +830,12 @@ __global__ __launch_bounds__(kBinThreads) __attribute__((amdgpu_waves_per_eu(3))
     }
 }
 
-// Kernel B. Same bin partition as kernel A; lane L of a wave handles bin
-// 64 w + L if this pass found a singleton there (sstamp == pass + 1), all of
-// the wave's singletons at once: the detection's k (held in registers) must
-// equal its slot's key; the last detection of the key queues the ball on its
-// bin in every group (the C x b bin digits are dot products of the k with the
-// M^T rows; one atomicExch per target) and stamps the target for the next
-// pass, whose kernel A subtracts it from the column it owns.
 constexpr int kKeyChunks = QSFT_MAX_N / 16;                        // 16-byte chunks of a k row
 
-__global__ __launch_bounds__(kBinThreads) void k_peel_apply(const qsft_peel_desc* __restrict__ descs,
-                                                            const int32_t* __restrict__ pre, int n_inst, int it) {
-    const int inst = find_inst(pre, n_inst, blockIdx.x);
-    const qsft_peel_desc& d = descs[inst];
-    uint32_t* __restrict__ ctr = d.ctr;
-    const int local = blockIdx.x - pre[inst];
-    if (it >= d.max_iter) return;
-    const uint32_t active = ctr[it * QSFT_CTR_ITER_STRIDE + 3];
-    if (local == 0 && threadIdx.x == 0) {
-        // loop control for the next pass (the counters of this pass are final)
-        const uint32_t ns = ctr[it * QSFT_CTR_ITER_STRIDE + 0];
-        const uint32_t nm = ctr[it * QSFT_CTR_ITER_STRIDE + 1];
-        const uint32_t nb = ctr[it * QSFT_CTR_ITER_STRIDE + 2];
-        uint32_t peeled = ctr[QSFT_CTR_PEELED];
-        if (active) {
-            ctr[QSFT_CTR_ITERS] = (uint32_t)(it + 1);
-            peeled = (peeled + nb < peeled) ? 0xFFFFFFFFu : peeled + nb;
-            ctr[QSFT_CTR_PEELED] = peeled;
-        }
-        const uint32_t next = (active && ns > 0 && nm > 0 && (double)peeled < d.peel_cap) ? 1u : 0u;
-        if (it + 1 < 16) ctr[(it + 1) * QSFT_CTR_ITER_STRIDE + 3] = next;
-    }
-    if (!active) return;
-    const int q = d.q, b = d.b, C = d.C, Q = d.Q, n_pad = d.n_pad;
-    const int nq = n_pad >> 4;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+// The checks and queue pushes of kernel B for the singleton found in bin t.
+__device__ __forceinline__ void apply_singleton(const qsft_peel_desc& d, int t, int it, int q, int b, int C, int Q,
+                                                int n_pad, int nq, uint32_t* ctr) {
     const int CQ = C * Q;
-    const int base = (local * kWaves + wave) * kBinsPerWave;
-    if (base >= CQ) return;
-    const int t = base + lane;
-    if (!(t < CQ && d.sstamp[t] == it + 1)) return;                // no cross-lane work below
     const ModQ mq((uint32_t)q);
     const int8_t* __restrict__ kcur = d.kbin + (size_t)(it & 1) * CQ * n_pad;
     const uint4* __restrict__ krow = reinterpret_cast<const uint4*>(kcur + (size_t)t * n_pad);
@@ -921,6 +895,45 @@ __global__ __launch_bounds__(kBinThreads) void k_peel_apply(const qsft_peel_desc
     }
 }
 
+// Kernel B. Same bin partition as kernel A; lane L of a wave handles bin
+// 64 w + L if this pass found a singleton there (sstamp == pass + 1), all of
+// the wave's singletons at once: the detection's k (held in registers) must
+// equal its slot's key; the last detection of the key queues the ball on its
+// bin in every group (the C x b bin digits are dot products of the k with the
+// M^T rows; one atomicExch per target) and stamps the target for the next
+// pass, whose kernel A subtracts it from the column it owns.
+
+__global__ __launch_bounds__(kBinThreads) void k_peel_apply(const qsft_peel_desc* __restrict__ descs,
+                                                            const int32_t* __restrict__ pre, int n_inst, int it) {
+    const int inst = find_inst(pre, n_inst, blockIdx.x);
+    const qsft_peel_desc& d = descs[inst];
+    uint32_t* __restrict__ ctr = d.ctr;
+    const int local = blockIdx.x - pre[inst];
+    if (it >= d.max_iter) return;
+    const uint32_t active = ctr[it * QSFT_CTR_ITER_STRIDE + 3];
+    if (local == 0 && threadIdx.x == 0) {
+        // loop control for the next pass (the counters of this pass are final)
+        const uint32_t ns = ctr[it * QSFT_CTR_ITER_STRIDE + 0];
+        const uint32_t nm = ctr[it * QSFT_CTR_ITER_STRIDE + 1];
+        const uint32_t nb = ctr[it * QSFT_CTR_ITER_STRIDE + 2];
+        uint32_t peeled = ctr[QSFT_CTR_PEELED];
+        if (active) {
+            ctr[QSFT_CTR_ITERS] = (uint32_t)(it + 1);
+            peeled = (peeled + nb < peeled) ? 0xFFFFFFFFu : peeled + nb;
+            ctr[QSFT_CTR_PEELED] = peeled;
+        }
+        const uint32_t next = (active && ns > 0 && nm > 0 && (double)peeled < d.peel_cap) ? 1u : 0u;
+        if (it + 1 < 16) ctr[(it + 1) * QSFT_CTR_ITER_STRIDE + 3] = next;
+    }
+    if (!active) return;
+    const int q = d.q, b = d.b, C = d.C, Q = d.Q, n_pad = d.n_pad;
+    const int nq = n_pad >> 4;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int CQ = C * Q;
+    const int t = (local * kWaves + wave) * kBinsPerWave + lane;
+    if (t < CQ && d.sstamp[t] == it + 1) apply_singleton(d, t, it, q, b, C, Q, n_pad, nq, ctr);   // no cross-lane work
+}
+
 // Output of one key: the running mean of its records in time order. Up to
 // kFinRecs records are gathered into registers by one walk of the key's list
 // and ordered there; keys with more records walk the list once per record.
@@ -928,21 +941,19 @@ constexpr int kFinRecs = 8;
 
 __global__ __launch_bounds__(kSlotThreads) void k_peel_finalize(const qsft_peel_desc* __restrict__ descs,
                                                                 const int32_t* __restrict__ pre, int n_inst) {
+    // one thread per claimed slot (the claim order of the registration), so
+    // the output row of a key is its position in that list
     const int inst = find_inst(pre, n_inst, blockIdx.x);
     const qsft_peel_desc& d = descs[inst];
-    const int H = 1 << d.hcap_log2;
-    const int h = (blockIdx.x - pre[inst]) * kSlotThreads + threadIdx.x;
-    const int lane = threadIdx.x & 63;
-    const uint32_t m = (h < H && d.tag[h] != 0) ? d.count[h] : 0u;
-    const uint64_t live = __ballot(m != 0);
-    if (live == 0) return;                                         // the whole wave
-    // output rows: one counter atomic per wave
-    const int first = __builtin_ctzll(live);
-    uint32_t obase = 0;
-    if (lane == first) obase = atomicAdd(&d.ctr[QSFT_CTR_OUT], (uint32_t)__popcll(live));
-    obase = (uint32_t)__shfl((int)obase, first);
-    if (m == 0) return;
-    const uint32_t o = obase + (uint32_t)__popcll(live & ((1ull << lane) - 1ull));
+    const uint32_t nlive = d.ctr[QSFT_CTR_LIVE];
+    const uint32_t o = (uint32_t)(blockIdx.x - pre[inst]) * kSlotThreads + threadIdx.x;
+    if (o == 0) {
+        d.ctr[QSFT_CTR_OUT] = nlive;
+        if (nlive > (uint32_t)d.out_cap) atomicOr(&d.ctr[QSFT_CTR_ERROR], (uint32_t)QSFT_DEVERR_OUT_FULL);
+    }
+    if (o >= nlive || o >= (uint32_t)d.out_cap) return;
+    const uint32_t h = d.live[o];
+    const uint32_t m = d.count[h];
     const uint32_t L = d.ctr[QSFT_CTR_LOG];
     // running mean over the records in time order: g <- (g * c + v) / (c + 1)
     double gx = 0.0, gy = 0.0;
@@ -1003,10 +1014,6 @@ __global__ __launch_bounds__(kSlotThreads) void k_peel_finalize(const qsft_peel_
             accumulate(cnt, d.rec_rho[2 * (size_t)bi], d.rec_rho[2 * (size_t)bi + 1]);
             prev = best;
         }
-    }
-    if ((int)o >= d.out_cap) {
-        atomicOr(&d.ctr[QSFT_CTR_ERROR], (uint32_t)QSFT_DEVERR_OUT_FULL);
-        return;
     }
     const int nw = d.n_pad >> 2;
     const uint32_t* ksrc = reinterpret_cast<const uint32_t*>(d.key + (size_t)h * d.n_pad);
@@ -1093,7 +1100,7 @@ int hcap_log2_for(long long CQ) {
     return l;
 }
 struct PeelLayout {
-    size_t off[20];
+    size_t off[21];
     size_t total;
 };
 PeelLayout peel_layout(long long CQ, int C, int n_pad, int hl2, int max_iter, int out_cap) {
@@ -1111,6 +1118,7 @@ PeelLayout peel_layout(long long CQ, int C, int n_pad, int hl2, int max_iter, in
         (size_t)out_cap * n_pad, (size_t)out_cap * 16, (size_t)out_cap * 8,  // 13-15 outputs
         (size_t)CQ * 4, (size_t)CQ * 4,                                      // 16-17 mark sstamp
         (size_t)CQ * 4, (size_t)CQ * 4 * C,                                  // 18-19 pend_head pend_next
+        H * 4,                                                               // 20 live
     };
     PeelLayout lay{};
     size_t o = 0;
@@ -1181,6 +1189,7 @@ extern "C" int qsft_peel_bind(qsft_peel_desc* h, int q, int n, int b, int C, int
     d.sstamp = reinterpret_cast<int32_t*>(base + lay.off[17]);
     d.pend_head = reinterpret_cast<int32_t*>(base + lay.off[18]);
     d.pend_next = reinterpret_cast<int32_t*>(base + lay.off[19]);
+    d.live = reinterpret_cast<uint32_t*>(base + lay.off[20]);
     d.flags = 0;
     d.energy = nullptr;
     *h = d;
@@ -1235,11 +1244,17 @@ extern "C" int qsft_peel_iteration(const qsft_peel_desc* d, const int32_t* pre_b
 }
 
 extern "C" int qsft_peel_finalize(const qsft_peel_desc* d, const int32_t* pre_slots, int n_inst, int blocks_slots,
-                                  void* stream) {
-    if (!d || !pre_slots || n_inst < 1 || blocks_slots < 1) { qsft_set_error("qsft_peel_finalize: bad arguments"); return QSFT_ERR_ARG; }
-    hipLaunchKernelGGL(k_peel_flush, dim3(blocks_slots), dim3(kSlotThreads), 0, (hipStream_t)stream, d, pre_slots, n_inst);
-    const hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return qsft_hip_status(e, "qsft_peel_finalize(flush)");
+                                  int word, void* stream) {
+    if (!d || !pre_slots || n_inst < 1 || blocks_slots < 1 || (word & ~QSFT_FIN_FLUSH)) {
+        qsft_set_error("qsft_peel_finalize: bad arguments");
+        return QSFT_ERR_ARG;
+    }
+    if (word & QSFT_FIN_FLUSH) {
+        hipLaunchKernelGGL(k_peel_flush, dim3(blocks_slots), dim3(kSlotThreads), 0, (hipStream_t)stream, d, pre_slots,
+                           n_inst);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return qsft_hip_status(e, "qsft_peel_finalize(flush)");
+    }
     hipLaunchKernelGGL(k_peel_finalize, dim3(blocks_slots), dim3(kSlotThreads), 0, (hipStream_t)stream, d, pre_slots, n_inst);
     return qsft_hip_status(hipGetLastError(), "qsft_peel_finalize");
 }
@@ -1250,7 +1265,7 @@ extern "C" int qsft_peel_run(const qsft_peel_desc* d, const int32_t* pre_bins, c
     int rc = qsft_peel_init(d, pre_bins, pre_slots, n_inst, blocks_bins, blocks_slots, stream);
     for (int it = 0; rc == QSFT_OK && it < max_iter; ++it)
         rc = qsft_peel_iteration(d, pre_bins, n_inst, blocks_bins, max_p, it, stream);
-    if (rc == QSFT_OK) rc = qsft_peel_finalize(d, pre_slots, n_inst, blocks_slots, stream);
+    if (rc == QSFT_OK) rc = qsft_peel_finalize(d, pre_slots, n_inst, blocks_slots, QSFT_FIN_FLUSH, stream);
     return rc;
 }
 

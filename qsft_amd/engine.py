@@ -104,7 +104,7 @@ class Batch:
     zeroton screen does not read U again."""
 
     def __init__(self, specs, precision="fp32", max_iter=15, device=None, eval_mode=None, fused_screen=True,
-                 host_rows=True, tail_stream=True):
+                 host_rows=True, tail_stream=True, own_stream=False):
         self.specs = list(specs)
         self.precision = precision
         if eval_mode is None:
@@ -147,19 +147,31 @@ class Batch:
         # packed rows: at most a few per planted support entry (noise adds some), kept
         # in pinned host memory that the pack kernels write directly
         cap = sum(4 * sp.S + 1024 for sp in self.specs) if host_rows else None
-        self.peel = K.PeelBatch(pl, max_iter=max_iter, host_rows=host_rows, cap_rows=cap)
+        self.peel = K.PeelBatch(pl, max_iter=max_iter, host_rows=host_rows, cap_rows=cap, final_u=False)
         # the peeling tail (last queued balls, output table, pack into host
         # memory) runs on a second stream, overlapping the next batch's work;
         # the next launch of this batch waits for it
-        self.tail = torch.cuda.Stream(device=dev) if (host_rows and tail_stream) else None
+        # With own_stream the whole pipeline runs on a stream of this batch, so
+        # the work of two resident batches interleaves on the GPU (the tail
+        # then stays on that stream).
+        self.stream = torch.cuda.Stream(device=dev) if own_stream else None
+        self.tail = torch.cuda.Stream(device=dev) if (host_rows and tail_stream and not own_stream) else None
         self.tail_event = torch.cuda.Event() if self.tail is not None else None
         self.done = None
 
     def launch(self, seed_offset=0, host_loop=False, events=None, marks=None):
-        """Enqueue the whole pipeline on the current stream. With `events`
-        (a dict), new HIP events bracket each stage for timing; `marks` maps
-        stage names ("start", "eval_prep", "eval", "dft", "assemble", "peel")
-        to pre-created events to record instead (no event creation per launch)."""
+        """Enqueue the whole pipeline on the batch's stream (own_stream) or the
+        current one. With `events` (a dict), new HIP events bracket each stage
+        for timing; `marks` maps stage names ("start", "eval_prep", "eval",
+        "dft", "assemble", "peel") to pre-created events to record instead (no
+        event creation per launch)."""
+        if self.stream is not None:
+            with torch.cuda.stream(self.stream):
+                self._enqueue(seed_offset, host_loop, events, marks)
+        else:
+            self._enqueue(seed_offset, host_loop, events, marks)
+
+    def _enqueue(self, seed_offset, host_loop, events, marks):
         def mark(name):
             if marks is not None and name in marks:
                 marks[name].record()

@@ -403,12 +403,15 @@ class PeelBatch:
     the same U).
     """
 
-    def __init__(self, items, max_iter=15, host_rows=False, cap_rows=None):
+    def __init__(self, items, max_iter=15, host_rows=False, cap_rows=None, final_u=True):
         """host_rows: the pack kernels write the header and rows straight into
         pinned host memory (read after the launch's completion event; no copy
         and no second stream). cap_rows: capacity of the packed rows (default:
-        every hash slot of every instance); overflow is reported as an error."""
+        every hash slot of every instance); overflow is reported as an error.
+        final_u: subtract the balls the last pass queued, so U ends as the
+        reference's loop leaves it (the recovered rows do not depend on it)."""
         self.items = items
+        self.fin_word = L.QSFT_FIN_FLUSH if final_u else 0
         self.max_iter = max_iter
         self.host_rows = host_rows
         self.descs = []
@@ -463,7 +466,7 @@ class PeelBatch:
         st = stream_handle()
         n = len(self.descs)
         d, pb, ps = _vp(self.dptr), _vp(self.pre_b), _vp(self.pre_s)
-        if not host_loop and tail_stream is None:
+        if not host_loop and tail_stream is None and self.fin_word == L.QSFT_FIN_FLUSH:
             L.check(lib.qsft_peel_run(d, pb, ps, n, self.tot_b, self.tot_s, self.max_p, self.max_iter, st),
                     "qsft_peel_run")
         else:
@@ -479,7 +482,7 @@ class PeelBatch:
                 tail_event.record()
                 tail_stream.wait_event(tail_event)
                 st = stream_handle(tail_stream)
-            L.check(lib.qsft_peel_finalize(d, ps, n, self.tot_s, st), "qsft_peel_finalize")
+            L.check(lib.qsft_peel_finalize(d, ps, n, self.tot_s, self.fin_word, st), "qsft_peel_finalize")
         L.check(lib.qsft_peel_pack(d, n, _vp(self.packed.data_ptr()), self.cap_rows, self.row_stride, st),
                 "qsft_peel_pack")
 
