@@ -35,6 +35,8 @@
 // subtiles; the fragments of the next three blocks are loaded straight into
 // LDS (a four-slot ring) while the current block's MFMAs run. Logical blocks are
 // dealt out XCD by XCD so the workgroups that share B fragments share an L2.
+#include <type_traits>
+
 #include "qsft_common.h"
 
 namespace qsft {
@@ -233,7 +235,11 @@ __device__ __forceinline__ void eval_mfma_body(const qsft_eval_mfma_desc* __rest
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int g = lane >> 4, r16 = lane & 15;
     const ModQ mq((uint32_t)q);
-    const uint32_t qshift = (q == 2) ? 1u : 0u;                   // MODE 0: exponent of i is E0 (q = 4) or 2 E0
+    // MODE 0 tables (see compute): q = 4: Re w^f = 1, 0, -1, 0 and Im w^f = 0, 1, 0, -1
+    // for f = 0..3; q = 2: Re w^f = 1, -1 for f = 0, 1
+    const uint32_t wmask = (q == 2) ? 0x01000100u : 0x03000300u;
+    const uint32_t wre = (q == 2) ? 0x0000BC3Cu : 0x00BC003Cu;
+    const uint32_t wim = (q == 2) ? 0u : 0xBC003C00u;
     const int lbase = lb * kRowsPerBlock + wave * 64;
 
     // base-query digit fragments (B operand of the int8 MFMA): column l = lbase + 16 t + r16
@@ -302,50 +308,65 @@ __device__ __forceinline__ void eval_mfma_body(const qsft_eval_mfma_desc* __rest
         }
     };
 
-    // one support block: W fragments from the int8 MFMA, then the f16 GEMM
-    auto compute = [&](const v4u* bcur, const v4i* kcur) {
+    // E0 tile of subtile t: support digits (A) x base-query digits (B), exact int32
+    auto e0_tile = [&](const v4i (&kf)[NKB], int t) {
+        v4i e = {0, 0, 0, 0};
+#pragma unroll
+        for (int kb = 0; kb < NKB; ++kb) e = __builtin_amdgcn_mfma_i32_16x16x64_i8(kf[kb], bq[t][kb], e, 0, 0, 0);
+        return e;
+    };
+    // W fragment(s) of one subtile from its E0 tile
+    auto w_tile = [&](const v4i e, v4u (&wt)[WL]) {
+        if constexpr (MODE == 0) {
+            // q in {2, 4}: w^E0 is 0 or +-1 in each of Re, Im, two supports per 32-bit word
+            // by byte permutes. The selector holds f = E0 mod q in bytes 1 and 3 (bytes 0 and
+            // 2 select a zero byte); the tables hold the high byte of the fp16 value of
+            // Re / Im w^f (1.0 = 0x3C00, -1.0 = 0xBC00) at byte f.
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const uint32_t sel =
+                    (__builtin_amdgcn_perm((uint32_t)e[2 * h + 1], (uint32_t)e[2 * h], 0x040C000Cu) & wmask) |
+                    0x000C000Cu;
+                wt[0][h] = __builtin_amdgcn_perm(0u, wre, sel);              // Re words 0, 1
+                wt[0][2 + h] = __builtin_amdgcn_perm(0u, wim, sel);          // Im words 2, 3
+            }
+        } else {
+            uint32_t r[4];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) r[v] = (MODE == 1) ? ((uint32_t)e[v] & (uint32_t)(q - 1)) : mq((uint32_t)e[v]);
+#pragma unroll
+            for (int li = 0; li < WL; ++li) {
+                uint32_t w[4];
+#pragma unroll
+                for (int v = 0; v < 4; ++v) w[v] = wtab[li][r[v]];
+                wt[li] = v4u{(w[0] & 0xFFFFu) | (w[1] << 16), (w[2] & 0xFFFFu) | (w[3] << 16),
+                             (w[0] >> 16) | (w[1] & 0xFFFF0000u), (w[2] >> 16) | (w[3] & 0xFFFF0000u)};
+            }
+        }
+    };
+    // W fragments of one support block
+    auto wbuild = [&](const v4i* kcur, v4u (&wa)[4][WL]) {
         v4i kf[NKB];
 #pragma unroll
         for (int kb = 0; kb < NKB; ++kb) kf[kb] = kcur[kb * 64 + lane];
-        // all four int8 products first, so their latency overlaps
-        v4i e4[4];
+        v4i e4[4];                                   // all four int8 products first, so their latency overlaps
 #pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            v4i e = {0, 0, 0, 0};
+        for (int t = 0; t < 4; ++t) e4[t] = e0_tile(kf, t);
 #pragma unroll
-            for (int kb = 0; kb < NKB; ++kb) e = __builtin_amdgcn_mfma_i32_16x16x64_i8(kf[kb], bq[t][kb], e, 0, 0, 0);
-            e4[t] = e;
-        }
-        v4u wa[4][WL];
+        for (int t = 0; t < 4; ++t) w_tile(e4[t], wa[t]);
+    };
+    // The f16 GEMM of one support block. With PIPE (MODE 0, needs NPT >= 5) the
+    // next block's W fragments (support digits kn) are built inside the first
+    // limb's MFMA sequence: its int8 products after p-tile 0, one subtile's
+    // byte permutes after each of p-tiles 1..4, so that VALU work runs while
+    // this wave's MFMAs execute; scheduling barriers keep that order.
+    auto gemm = [&](const v4u* bcur, const v4u (&wa)[4][WL], const v4i* kn, v4u (&wn)[4][WL], auto pipe) {
+        constexpr bool PIPE = decltype(pipe)::value;
+        v4i kfn[NKB];
+        v4i en[4];
+        if constexpr (PIPE) {
 #pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            const v4i e = e4[t];
-            if constexpr (MODE == 0) {
-                // q in {2, 4}, two supports per 32-bit word (E0 >= 0 and < 2^16, so the halves do not mix).
-                // With f = 2 E0 (q = 2) or E0 (q = 4), w^E0 = i^f: Re = {1, 0, -1, 0}, Im = {0, 1, 0, -1}
-                // over f mod 4, i.e. magnitude fp16 1.0 (0x3C00), sign from bit 1, in Re (bit 0 clear) or Im.
-#pragma unroll
-                for (int h = 0; h < 2; ++h) {
-                    const uint32_t pk = (((uint32_t)e[2 * h + 1] << 16) | ((uint32_t)e[2 * h] & 0xFFFFu)) << qshift;
-                    const uint32_t sgn = ((pk & 0x00020002u) << 14) | 0x3C003C00u;     // +-1.0 in each half
-                    const uint32_t m = (pk & 0x00010001u) * 0xFFFFu;                  // 0xFFFF where Im
-                    const uint32_t im = sgn & m;
-                    wa[t][0][h] = sgn ^ im;                                         // Re words 0, 1
-                    wa[t][0][2 + h] = im;                                           // Im words 2, 3
-                }
-            } else {
-                uint32_t r[4];
-#pragma unroll
-                for (int v = 0; v < 4; ++v) r[v] = (MODE == 1) ? ((uint32_t)e[v] & (uint32_t)(q - 1)) : mq((uint32_t)e[v]);
-#pragma unroll
-                for (int li = 0; li < WL; ++li) {
-                    uint32_t w[4];
-#pragma unroll
-                    for (int v = 0; v < 4; ++v) w[v] = wtab[li][r[v]];
-                    wa[t][li] = v4u{(w[0] & 0xFFFFu) | (w[1] << 16), (w[2] & 0xFFFFu) | (w[3] << 16),
-                                    (w[0] >> 16) | (w[1] & 0xFFFF0000u), (w[2] >> 16) | (w[3] & 0xFFFF0000u)};
-                }
-            }
+            for (int kb = 0; kb < NKB; ++kb) kfn[kb] = kn[kb * 64 + lane];
         }
         // limb-major order: an accumulator is revisited only every 8 x 4 MFMAs
 #pragma unroll
@@ -364,6 +385,20 @@ __device__ __forceinline__ void eval_mfma_body(const qsft_eval_mfma_desc* __rest
                     for (int t = 0; t < 4; ++t)
                         acc[t][pt] = __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(v8h, wa[t][wl]), bf,
                                                                           acc[t][pt], 0, 0, 0);
+                    if constexpr (PIPE) {
+                        if (bl == 0) {
+                            if (pt == 0) {
+#pragma unroll
+                                for (int t = 0; t < 4; ++t) en[t] = e0_tile(kfn, t);
+                            } else if (pt <= 4) {
+                                w_tile(en[pt - 1], wn[pt - 1]);
+#pragma unroll
+                                for (int li = 0; li < WL; ++li)       // materialise here: keeps the
+                                    __asm__ volatile("" : "+v"(wn[pt - 1][li]));   // permutes in this slot
+                            }
+                        }
+                        __builtin_amdgcn_sched_barrier(0);
+                    }
                 }
             }
         }
@@ -380,16 +415,29 @@ __device__ __forceinline__ void eval_mfma_body(const qsft_eval_mfma_desc* __rest
     wait_lgkm0();                                    // this wave's wtab writes
     ring_barrier();                                  // ... for every wave; wtab visible
 
+    // a pair of blocks: the second block's W fragments are built while the
+    // first block's MFMAs run (independent instructions the scheduler interleaves)
+    v4u w0[4][WL], w1[4][WL];
+    constexpr bool kPipe = MODE == 0 && NPT >= 5;
+    auto compute_pair = [&](const v4u* b0, const v4i* k0, const v4u* b1, const v4i* k1, bool two) {
+        wbuild(k0, w0);
+        if constexpr (kPipe) {
+            // the second slot's digits are read even when it holds no block (its W is then unused)
+            gemm(b0, w0, k1, w1, std::true_type{});
+        } else {
+            if (two) wbuild(k1, w1);
+            gemm(b0, w0, k1, w1, std::false_type{});
+        }
+        if (two) gemm(b1, w1, k1, w1, std::false_type{});
+    };
     for (int sb = 0; sb < nsb; sb += 4) {
-        compute(bs0, ks0);
-        if (sb + 1 < nsb) compute(bs1, ks1);
+        compute_pair(bs0, ks0, bs1, ks1, sb + 1 < nsb);
         if (sb + 2 >= nsb) break;
         wait_groups(0);                              // pair B (blocks sb + 2, sb + 3) has landed
         ring_barrier();                              // ... for every wave; pair A is free
         if (sb + 4 < nsb) issue(sb + 4, bs0, ks0);
         if (sb + 5 < nsb) issue(sb + 5, bs1, ks1);
-        compute(bs2, ks2);
-        if (sb + 3 < nsb) compute(bs3, ks3);
+        compute_pair(bs2, ks2, bs3, ks3, sb + 3 < nsb);
         if (sb + 4 >= nsb) break;
         wait_groups(0);                              // pair A (blocks sb + 4, sb + 5) has landed
         ring_barrier();                              // ... for every wave; pair B is free
