@@ -43,8 +43,8 @@ PEAK_F16_TFLOPS = 2500.0         # MI355X dense fp16 / bf16 MFMA, TFLOP/s
 def parse():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=40)
+    ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--plain", type=int, default=3, help="plain FP64 transforms to time")
     ap.add_argument("--seed", type=int, default=1234)

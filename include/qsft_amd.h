@@ -328,7 +328,10 @@ int qsft_peel_finalize(const qsft_peel_desc* dev_descs, const int32_t* dev_pre_s
 /* Gather every instance's recovered rows into one buffer `dst` (device or
  * host-mapped) so a batch comes back in one copy: an int64 header of
  * qsft_peel_pack_header_bytes(n_inst) bytes holding counts[n_inst],
- * offsets[n_inst], total, error bits; then rows of row_stride bytes:
+ * offsets[n_inst], total, error bits, the most passes any instance ran, and
+ * the number of instances whose loop asked for a pass beyond those launched
+ * (a caller that launched fewer than max_iter passes runs the rest and
+ * finalizes and packs again); then rows of row_stride bytes:
  * k digits (the instance's n_pad bytes), value (2 doubles), first detection
  * (uint64). Rows beyond cap_rows are dropped and flagged QSFT_DEVERR_OUT_FULL.
  * Replaces the result dict assembly of qsft/qsft.py:247-255. */

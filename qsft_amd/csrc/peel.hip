@@ -548,7 +548,8 @@ __global__ __launch_bounds__(kBinThreads) __attribute__((amdgpu_waves_per_eu(3))
     const qsft_peel_desc& d = descs[inst];
     uint32_t* __restrict__ ctr = d.ctr;
     if (it >= d.max_iter) return;                                  // beyond the bound workspace
-    if (ctr[it * QSFT_CTR_ITER_STRIDE + 3] == 0) return;          // loop already stopped
+    if (cptr(ctr)[it * QSFT_CTR_ITER_STRIDE + 3] == 0) return;    // loop already stopped (set by the
+                                                                   // previous pass; scalar load)
     const int q = d.q, n = d.n, b = d.b, C = d.C, P = d.P, Q = d.Q, n_pad = d.n_pad;
     const int nw = n_pad >> 2;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -910,7 +911,7 @@ __global__ __launch_bounds__(kBinThreads) void k_peel_apply(const qsft_peel_desc
     uint32_t* __restrict__ ctr = d.ctr;
     const int local = blockIdx.x - pre[inst];
     if (it >= d.max_iter) return;
-    const uint32_t active = ctr[it * QSFT_CTR_ITER_STRIDE + 3];
+    const uint32_t active = cptr(ctr)[it * QSFT_CTR_ITER_STRIDE + 3];   // written before this kernel
     if (local == 0 && threadIdx.x == 0) {
         // loop control for the next pass (the counters of this pass are final)
         const uint32_t ns = ctr[it * QSFT_CTR_ITER_STRIDE + 0];
@@ -1025,12 +1026,15 @@ __global__ __launch_bounds__(kSlotThreads) void k_peel_finalize(const qsft_peel_
 }
 
 // Exclusive scan of the per-instance output counts into the pack header:
-// hdr[i] = count_i, hdr[n + i] = offset_i, hdr[2n] = total, hdr[2n + 1] = OR of error bits.
+// hdr[i] = count_i, hdr[n + i] = offset_i, hdr[2n] = total, hdr[2n + 1] = OR of
+// error bits, hdr[2n + 2] = most passes run, hdr[2n + 3] = instances whose loop
+// asked for a pass beyond those launched.
 __global__ __launch_bounds__(64) void k_pack_scan(const qsft_peel_desc* __restrict__ descs, int n_inst,
                                                   int64_t* __restrict__ hdr, int64_t cap_rows) {
     const int lane = threadIdx.x;
     long long carry = 0;
     uint32_t err = 0;
+    int iters = 0, unfinished = 0;
     for (int i0 = 0; i0 < n_inst; i0 += 64) {                     // 64 instances per step, lane-parallel
         const int i = i0 + lane;
         long long m = 0;
@@ -1040,6 +1044,10 @@ __global__ __launch_bounds__(64) void k_pack_scan(const qsft_peel_desc* __restri
             if ((int)c > d.out_cap) c = (uint32_t)d.out_cap;
             err |= d.ctr[QSFT_CTR_ERROR];
             m = c;
+            // passes that ran, and whether the loop asked for one more than were launched
+            const int it = (int)d.ctr[QSFT_CTR_ITERS];
+            iters = max(iters, it);
+            if (it < d.max_iter && d.ctr[it * QSFT_CTR_ITER_STRIDE + 3] != 0) ++unfinished;
         }
         long long incl = m;                                        // inclusive scan over the lanes
 #pragma unroll
@@ -1054,11 +1062,17 @@ __global__ __launch_bounds__(64) void k_pack_scan(const qsft_peel_desc* __restri
         carry += __shfl(incl, 63);
     }
 #pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) err |= (uint32_t)__shfl_xor((int)err, off);
+    for (int off = 32; off >= 1; off >>= 1) {
+        err |= (uint32_t)__shfl_xor((int)err, off);
+        iters = max(iters, __shfl_xor(iters, off));
+        unfinished += __shfl_xor(unfinished, off);
+    }
     if (lane == 0) {
         if (carry > cap_rows) err |= QSFT_DEVERR_OUT_FULL;
         hdr[2 * n_inst] = carry;
         hdr[2 * n_inst + 1] = err;
+        hdr[2 * n_inst + 2] = iters;
+        hdr[2 * n_inst + 3] = unfinished;
     }
 }
 
@@ -1270,7 +1284,7 @@ extern "C" int qsft_peel_run(const qsft_peel_desc* d, const int32_t* pre_bins, c
 }
 
 extern "C" size_t qsft_peel_pack_header_bytes(int n_inst) {
-    return n_inst < 1 ? 0 : (((size_t)(2 * n_inst + 2) * 8 + 255) & ~(size_t)255);
+    return n_inst < 1 ? 0 : (((size_t)(2 * n_inst + 4) * 8 + 255) & ~(size_t)255);
 }
 
 extern "C" int qsft_peel_pack(const qsft_peel_desc* d, int n_inst, void* dst, int64_t cap_rows, int row_stride,

@@ -24,12 +24,15 @@ typedef double gd2 __attribute__((ext_vector_type(2)));
 typedef int gi4 __attribute__((ext_vector_type(4)));
 typedef unsigned int gu4 __attribute__((ext_vector_type(4)));
 
-// Largest i with pre[i] <= blk: the instance that owns work block blk.
+// Largest i with pre[i] <= blk: the instance that owns work block blk. The
+// prefix array is written by the host before the launch, so the search reads
+// it with scalar loads (scalar cache).
 __device__ __forceinline__ int find_inst(const int32_t* __restrict__ pre, int n_inst, int blk) {
+    const __attribute__((address_space(4))) int32_t* cp = (const __attribute__((address_space(4))) int32_t*)pre;
     int lo = 0, hi = n_inst - 1;
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
-        if (pre[mid] <= blk) lo = mid; else hi = mid - 1;
+        if (cp[mid] <= blk) lo = mid; else hi = mid - 1;
     }
     return lo;
 }

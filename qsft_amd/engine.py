@@ -14,6 +14,7 @@ workspace per instance, builds every stage's descriptors once, and then
 of all recovered rows. Uneven instances (different q, n, b, S, C, R) share
 the same launches.
 """
+import collections
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -158,6 +159,15 @@ class Batch:
         self.tail = torch.cuda.Stream(device=dev) if (host_rows and tail_stream and not own_stream) else None
         self.tail_event = torch.cuda.Event() if self.tail is not None else None
         self.done = None
+        # Passes launched per batch: one more than most recent launches ran (the
+        # 80th percentile of the last 16). Passes after the loop stops only cost
+        # launches; a launch that falls short (an instance whose loop runs to
+        # the pass cap) is finished in fetch(), as the packed header says.
+        self.max_iter = max_iter
+        self.adaptive = self.tail is not None or self.stream is not None
+        self.passes = max_iter
+        self.launched = max_iter
+        self._ran = collections.deque(maxlen=16)
 
     def launch(self, seed_offset=0, host_loop=False, events=None, marks=None):
         """Enqueue the whole pipeline on the batch's stream (own_stream) or the
@@ -191,7 +201,8 @@ class Batch:
         mark("dft")
         self.asm.launch(seed_offset)
         mark("assemble")
-        self.peel.run(host_loop=host_loop, tail_stream=self.tail, tail_event=self.tail_event)
+        self.launched = self.passes if (self.adaptive and not host_loop) else self.max_iter
+        self.peel.run(host_loop=host_loop, tail_stream=self.tail, tail_event=self.tail_event, passes=self.launched)
         # one completion event per batch, re-recorded every launch (a copy that
         # waits on it captured the previous record when it was enqueued)
         if self.done is None:
@@ -211,7 +222,24 @@ class Batch:
         """Packed rows of the last launch once it is done: (counts, offsets, rows).
         Work queued after it keeps running. With host_rows the rows are already
         in pinned host memory; otherwise they are copied on `copy_stream`."""
-        return self.peel.fetch_packed(copy_stream=copy_stream, ready=self.done)
+        packed = self.peel.fetch_packed(copy_stream=copy_stream, ready=self.done)
+        if self.peel.last_unfinished and self.launched < self.max_iter:
+            # a loop wanted more passes than were launched: run them, then
+            # finalize and pack again
+            stream = self.stream if self.stream is not None else torch.cuda.current_stream()
+            with torch.cuda.stream(stream):
+                self.peel.resume(self.launched, tail_stream=self.tail, tail_event=self.tail_event)
+                if self.tail is not None:
+                    with torch.cuda.stream(self.tail):
+                        self.done.record()
+                else:
+                    self.done.record()
+            self.launched = self.max_iter
+            packed = self.peel.fetch_packed(copy_stream=copy_stream, ready=self.done)
+        if self.adaptive:
+            self._ran.append(self.peel.last_iters)
+            self.passes = min(self.max_iter, int(np.percentile(list(self._ran), 80)) + 1)
+        return packed
 
     def unpack(self, packed, with_info=False):
         return self.peel.results(with_info=with_info, packed=packed)

@@ -458,31 +458,53 @@ class PeelBatch:
             self.packed = torch.empty(size, dtype=torch.uint8, device=dev)
             self.hdr_host = torch.empty(self.hdr_bytes, dtype=torch.uint8).pin_memory()
 
-    def run(self, host_loop=False, tail_stream=None, tail_event=None):
+    def run(self, host_loop=False, tail_stream=None, tail_event=None, passes=None):
         """The peeling passes on the current stream, then the tail (last
         pass's queued balls, output table, pack). With `tail_stream` the tail
         runs there, after `tail_event` (recorded here on the current stream),
-        so the current stream can go on with other work meanwhile."""
+        so the current stream can go on with other work meanwhile. `passes`
+        (< max_iter) launches only that many passes; the packed header then
+        says whether a loop wanted more (resume() runs the rest)."""
         st = stream_handle()
         n = len(self.descs)
         d, pb, ps = _vp(self.dptr), _vp(self.pre_b), _vp(self.pre_s)
-        if not host_loop and tail_stream is None and self.fin_word == L.QSFT_FIN_FLUSH:
+        passes = self.max_iter if passes is None else min(int(passes), self.max_iter)
+        if (not host_loop and tail_stream is None and self.fin_word == L.QSFT_FIN_FLUSH
+                and passes == self.max_iter):
             L.check(lib.qsft_peel_run(d, pb, ps, n, self.tot_b, self.tot_s, self.max_p, self.max_iter, st),
                     "qsft_peel_run")
-        else:
-            L.check(lib.qsft_peel_init(d, pb, ps, n, self.tot_b, self.tot_s, st), "qsft_peel_init")
-            for it in range(self.max_iter):
-                L.check(lib.qsft_peel_iteration(d, pb, n, self.tot_b, self.max_p, it, st), "qsft_peel_iteration")
-                if not host_loop or it + 1 >= self.max_iter:
-                    continue
-                # host-driven loop: one sync per pass to read the stop flag (the plain path)
-                if not any(self._ctr_word(i, (it + 1) * L.QSFT_CTR_ITER_STRIDE + 3) for i in range(n)):
-                    break
-            if tail_stream is not None:
-                tail_event.record()
-                tail_stream.wait_event(tail_event)
-                st = stream_handle(tail_stream)
-            L.check(lib.qsft_peel_finalize(d, ps, n, self.tot_s, self.fin_word, st), "qsft_peel_finalize")
+            L.check(lib.qsft_peel_pack(d, n, _vp(self.packed.data_ptr()), self.cap_rows, self.row_stride, st),
+                    "qsft_peel_pack")
+            return
+        L.check(lib.qsft_peel_init(d, pb, ps, n, self.tot_b, self.tot_s, st), "qsft_peel_init")
+        for it in range(passes):
+            L.check(lib.qsft_peel_iteration(d, pb, n, self.tot_b, self.max_p, it, st), "qsft_peel_iteration")
+            if not host_loop or it + 1 >= self.max_iter:
+                continue
+            # host-driven loop: one sync per pass to read the stop flag (the plain path)
+            if not any(self._ctr_word(i, (it + 1) * L.QSFT_CTR_ITER_STRIDE + 3) for i in range(n)):
+                break
+        self._tail(tail_stream, tail_event)
+
+    def resume(self, start, tail_stream=None, tail_event=None):
+        """Passes start .. max_iter - 1 (each stops at once if its loop already
+        has), then the tail again: for a run() that launched too few passes."""
+        st = stream_handle()
+        n = len(self.descs)
+        for it in range(start, self.max_iter):
+            L.check(lib.qsft_peel_iteration(_vp(self.dptr), _vp(self.pre_b), n, self.tot_b, self.max_p, it, st),
+                    "qsft_peel_iteration")
+        self._tail(tail_stream, tail_event)
+
+    def _tail(self, tail_stream, tail_event):
+        n = len(self.descs)
+        d, ps = _vp(self.dptr), _vp(self.pre_s)
+        st = stream_handle()
+        if tail_stream is not None:
+            tail_event.record()
+            tail_stream.wait_event(tail_event)
+            st = stream_handle(tail_stream)
+        L.check(lib.qsft_peel_finalize(d, ps, n, self.tot_s, self.fin_word, st), "qsft_peel_finalize")
         L.check(lib.qsft_peel_pack(d, n, _vp(self.packed.data_ptr()), self.cap_rows, self.row_stride, st),
                 "qsft_peel_pack")
 
@@ -532,8 +554,9 @@ class PeelBatch:
         rows = self.rows_host[:nbytes].numpy().copy()
         return hdr[:n].copy(), hdr[n:2 * n].copy(), rows.reshape(total, self.row_stride)
 
-    @staticmethod
-    def _check_header(hdr, n):
+    def _check_header(self, hdr, n):
+        self.last_iters = int(hdr[2 * n + 2])                      # most passes any instance ran
+        self.last_unfinished = int(hdr[2 * n + 3])                  # instances that wanted more passes
         err = int(hdr[2 * n + 1])
         if err:
             raise L.QsftLibraryError(f"peel device error bits {err:#x} (1 table full, 2 tag collision, "

@@ -105,3 +105,23 @@ def test_host_rows_match_device_copy():
     b.launch(seed_offset=5)
     for x, y in zip(a.unpack(a.fetch()), b.unpack(b.fetch(torch.cuda.Stream()))):
         _same((x[0], x[1]), (y[0], y[1]), rtol=0.0)
+
+
+def test_short_pass_launch_is_finished_on_fetch():
+    """A launch with fewer peeling passes than its loops need (the served batches
+    launch the most recent batches needed, plus one) is finished in fetch():
+    the rows equal those of a launch with every pass."""
+    rng = np.random.default_rng(21)
+    specs = [E.random_spec(4, 30, 200, 4, 3, 2, 15.0, rng=rng), E.random_spec(4, 20, 120, 4, 3, 2, None, rng=rng)]
+    full = E.Batch(specs, precision="fp32")
+    short = E.Batch(specs, precision="fp32")
+    full.launch(seed_offset=3)
+    ref = full.results()
+    short.passes = 1
+    short.launch(seed_offset=3)
+    got = short.results()
+    assert short.launched == short.max_iter                          # the rest of the passes ran in fetch()
+    for (ka, va, oa, _), (kb, vb, ob, _) in zip(got, ref):
+        assert np.array_equal(ka, kb) and np.array_equal(oa, ob)
+        assert np.array_equal(va, vb)
+    assert short.passes <= short.max_iter and short.peel.last_iters >= 2
