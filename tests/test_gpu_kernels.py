@@ -63,7 +63,11 @@ def test_eval_cosets_matches_oracle(q, n, b, S, C, R, precision):
             assert rel_err(x[c, p], ref) < tol, (c, p)
 
 
-MFMA_CASES = CASES + [(8, 30, 3, 77, 2, 2), (2, 250, 4, 33, 3, 1), (4, 64, 5, 129, 3, 2)]
+# q in {2, 4} with n <= 64 (the two-waves-per-SIMD kernel): P = 72 is nine
+# 8-row tiles (two workgroups of 5 tiles, the last one partly empty), P = 82
+# eleven (6 + 5); q = 2 exercises the one-bit W selector
+MFMA_CASES = CASES + [(8, 30, 3, 77, 2, 2), (2, 250, 4, 33, 3, 1), (4, 64, 5, 129, 3, 2), (4, 35, 4, 150, 2, 2),
+                      (2, 40, 5, 77, 2, 2)]
 
 
 @pytest.mark.parametrize("q,n,b,S,C,R", MFMA_CASES)
