@@ -202,6 +202,11 @@ typedef struct qsft_assemble_desc {
     double* energy;         /* optional [C][Q]: |U column|^2 of every bin as the peeler     *
                              * computes it (bin-major and R*p1 <= 128 only), for its      *
                              * first-pass zeroton screen; NULL: not written                */
+    uint8_t* written;       /* optional [C][Q] (needs energy): write a bin's column only    *
+                             * if its energy exceeds cutoff * R*p1 (the peeler's zeroton   *
+                             * test) and store 1 / 0 here; NULL: every column is written   */
+    double cutoff;          /* with `written`: the peeler's cutoff                          */
+    uint64_t* seed_out;     /* optional: the Philox key of the launch (seed + seed_offset)  */
 } qsft_assemble_desc;
 
 int qsft_assemble_blocks(const qsft_assemble_desc* host_desc);
@@ -274,6 +279,15 @@ typedef struct qsft_peel_desc {
     int8_t* out_k;          /* [out_cap][n_pad] recovered k                                 */
     double* out_val;        /* [out_cap][2]     running-mean coefficient                    */
     uint64_t* out_order;    /* [out_cap]        first (iteration, bin) of k: dict order     */
+    /* sparse working transform (QSFT_PEEL_SPARSE_U; the caller sets these after
+     * qsft_peel_bind and points the assembler's written / seed_out at `written`
+     * / `noise_key`): U holds only the columns the assembler wrote; a column it
+     * skipped (a first-pass zeroton) is rebuilt from the samples and the same
+     * Philox noise when a ball first lands on it, and written back then. */
+    const void* xsrc;       /* [C][P][Q] complex FP32 samples, the assembler's source       */
+    double noise_sigma;     /* the assembler's sigma (0: no noise)                          */
+    uint8_t* written;       /* [C*Q] 1: the bin's column is in U (workspace)                */
+    uint64_t* noise_key;    /* the assembler's Philox key of this launch (workspace)        */
 } qsft_peel_desc;
 
 /* qsft_peel_desc::flags */
@@ -283,6 +297,7 @@ typedef struct qsft_peel_desc {
                                  * and the identity delays (s = +1); signatures then take *
                                  * R dot products per key instead of P                   */
 #define QSFT_PEEL_UNIT_D_NEG 4  /* with QSFT_PEEL_UNIT_D: s = -1                          */
+#define QSFT_PEEL_SPARSE_U 8    /* U holds only the columns the assembler wrote (above)  */
 
 /* counter layout (uint32 words); per iteration i at 8 i + :
  *   0 singletons, 1 multitons, 2 distinct balls, 3 active flag,
@@ -320,7 +335,8 @@ int qsft_peel_iteration(const qsft_peel_desc* dev_descs, const int32_t* dev_pre_
                         int blocks_bins, int max_p, int iteration, void* stream);
 /* word: QSFT_FIN_FLUSH first subtracts the balls the last pass queued, so U
  * ends as the reference's loop leaves it; without it U keeps them queued
- * (the recovered rows are the same either way). */
+ * (the recovered rows are the same either way). With QSFT_PEEL_SPARSE_U the
+ * columns that were never written stay unwritten (samples plus noise). */
 #define QSFT_FIN_FLUSH 1
 int qsft_peel_finalize(const qsft_peel_desc* dev_descs, const int32_t* dev_pre_slots, int n_inst,
                        int blocks_slots, int word, void* stream);

@@ -22,7 +22,7 @@ QSFT_NOISE_NONE, QSFT_NOISE_ARRAY, QSFT_NOISE_PHILOX = 0, 1, 2
 QSFT_DEVERR_TABLE_FULL, QSFT_DEVERR_TAG_COLLISION, QSFT_DEVERR_OUT_FULL = 1, 2, 4
 QSFT_MAX_Q, QSFT_MAX_N, QSFT_MAX_B, QSFT_MAX_C, QSFT_MAX_R = 64, 256, 16, 16, 16
 QSFT_DFT_MAX_PASSES = 4
-QSFT_PEEL_SHARED_D, QSFT_PEEL_UNIT_D, QSFT_PEEL_UNIT_D_NEG = 1, 2, 4
+QSFT_PEEL_SHARED_D, QSFT_PEEL_UNIT_D, QSFT_PEEL_UNIT_D_NEG, QSFT_PEEL_SPARSE_U = 1, 2, 4, 8
 QSFT_CTR_ITER_STRIDE = 8
 QSFT_CTR_LOG, QSFT_CTR_OUT, QSFT_CTR_ERROR, QSFT_CTR_ITERS, QSFT_CTR_PEELED, QSFT_CTR_LIVE = 128, 129, 130, 131, 132, 133
 QSFT_FIN_FLUSH = 1
@@ -53,7 +53,8 @@ class AssembleDesc(ctypes.Structure):
     _fields_ = [("Q", i32), ("C", i32), ("R", i32), ("p1", i32), ("src_rows", i32), ("src_prec", i32),
                 ("noise_mode", i32), ("bin_major", i32),
                 ("gsel", i32 * QSFT_MAX_C), ("rsel", i32 * QSFT_MAX_R),
-                ("src", vp), ("noise", vp), ("sigma", f64), ("seed", u64), ("U", vp), ("energy", vp)]
+                ("src", vp), ("noise", vp), ("sigma", f64), ("seed", u64), ("U", vp), ("energy", vp),
+                ("written", vp), ("cutoff", f64), ("seed_out", vp)]
 
 
 class PeelDesc(ctypes.Structure):
@@ -67,7 +68,8 @@ class PeelDesc(ctypes.Structure):
                 ("tag", vp), ("owner", vp),
                 ("last", vp), ("head", vp), ("count", vp), ("live", vp), ("key", vp), ("rec_time", vp),
                 ("rec_rho", vp), ("rec_next", vp), ("ctr", vp),
-                ("out_k", vp), ("out_val", vp), ("out_order", vp)]
+                ("out_k", vp), ("out_val", vp), ("out_order", vp),
+                ("xsrc", vp), ("noise_sigma", f64), ("written", vp), ("noise_key", vp)]
 
 
 class EvalMfmaDesc(ctypes.Structure):

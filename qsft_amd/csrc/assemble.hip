@@ -23,11 +23,6 @@ constexpr int kAsmPerThread = 4;
 constexpr int kBmBins = 16;        // bins per bin-major tile
 constexpr int kBmRows = 128;       // rows per bin-major tile (row chunks beyond)
 
-// Counter of the Philox block shared by bins 2h and 2h + 1 of row (c, p).
-__device__ __forceinline__ uint64_t noise_counter(int c, int p, int rows, int64_t Q, int64_t j) {
-    return ((uint64_t)c * rows + p) * (uint64_t)((Q + 1) >> 1) + (uint64_t)(j >> 1);
-}
-
 __device__ __forceinline__ double2 src_value(const qsft_assemble_desc& d, int c, int p, int64_t j) {
     const int r = p / d.p1, i = p - r * d.p1;
     const int64_t src_idx = ((int64_t)d.gsel[c] * d.src_rows + (int64_t)d.rsel[r] * d.p1 + i) * d.Q + j;
@@ -66,6 +61,7 @@ __global__ __launch_bounds__(kAsmThreads) void k_assemble(const qsft_assemble_de
     const int local = blockIdx.x - pre[inst];
     const double comp_sd = d.sigma * 0.70710678118654752440;      // per component: sqrt(sigma^2 / 2)
     const uint64_t seed = d.seed + seed_offset;
+    if (d.seed_out && local == 0 && threadIdx.x == 0) *d.seed_out = seed;
     if (!d.bin_major) {
         const int64_t total = (int64_t)d.C * rows * Q;
         const int64_t e0 = (int64_t)local * (kAsmThreads * kAsmPerThread) + threadIdx.x;
@@ -77,11 +73,7 @@ __global__ __launch_bounds__(kAsmThreads) void k_assemble(const qsft_assemble_de
             const int c = (int)(rowg / rows);
             const int p = (int)(rowg - (int64_t)c * rows);
             double2 v = add_array_noise(d, src_value(d, c, p, j), e);
-            if (d.noise_mode == QSFT_NOISE_PHILOX) {
-                const float4 z = normal4(noise_counter(c, p, rows, Q, j), seed);
-                v.x = v.x + comp_sd * (double)((j & 1) ? z.z : z.x);
-                v.y = v.y + comp_sd * (double)((j & 1) ? z.w : z.y);
-            }
+            if (d.noise_mode == QSFT_NOISE_PHILOX) v = add_philox(v, c, p, rows, Q, j, comp_sd, seed);
             reinterpret_cast<double2*>(d.U)[e] = v;
         }
         return;
@@ -146,11 +138,14 @@ __global__ __launch_bounds__(kAsmThreads) void k_assemble(const qsft_assemble_de
     __syncthreads();
     // write: wave w takes bin pairs (2h, 2h + 1), h = w, w + 4, ...; lane L holds
     // rows L and L + 64 of the chunk (the peeler's column layout), so each store
-    // is 1 KiB contiguous and the energies reduce straight from registers
+    // is 1 KiB contiguous and the energies reduce straight from registers. With
+    // `written` (energies, so all rows are in this chunk) a column is stored
+    // only if its energy is above the peeler's zeroton cutoff.
     double2* __restrict__ U = reinterpret_cast<double2*>(d.U);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int npairs = (Jc + 1) >> 1;
     const bool philox = d.noise_mode == QSFT_NOISE_PHILOX;
+    const double thr = d.cutoff * (double)rows;                    // as the peeler's screen forms it
     for (int h = wave; h < npairs; h += kAsmThreads / 64) {
         const int jj = 2 * h;
         const int64_t j = j0 + jj;
@@ -170,22 +165,36 @@ __global__ __launch_bounds__(kAsmThreads) void k_assemble(const qsft_assemble_de
                 double2 v1 = two ? to_d2(tile[pr][jj + 1]) : make_double2(0.0, 0.0);
                 if (philox) {
                     const float4 z = normal4(noise_counter(c, p, rows, Q, j), seed);
-                    v0.x = v0.x + comp_sd * (double)z.x;
-                    v0.y = v0.y + comp_sd * (double)z.y;
-                    v1.x = v1.x + comp_sd * (double)z.z;
-                    v1.y = v1.y + comp_sd * (double)z.w;
+                    v0 = noise_add(v0, z.x, z.y, comp_sd);
+                    v1 = noise_add(v1, z.z, z.w, comp_sd);
                 }
-                U[((int64_t)c * Q + j) * rows + p] = v0;
-                if (two) U[((int64_t)c * Q + j + 1) * rows + p] = v1;
                 u0[r] = v0;
                 u1[r] = v1;
             }
         }
+        bool w0 = true, w1 = two;
         if (d.energy) {                          // rows <= kBmRows here (checked on the host)
             const double e0 = bin_energy(u0), e1 = bin_energy(u1);
+            if (d.written) {
+                w0 = e0 > thr;
+                w1 = two && e1 > thr;
+            }
             if (lane == 0) {
                 d.energy[(int64_t)c * Q + j] = e0;
                 if (two) d.energy[(int64_t)c * Q + j + 1] = e1;
+                if (d.written) {
+                    d.written[(int64_t)c * Q + j] = w0 ? 1 : 0;
+                    if (two) d.written[(int64_t)c * Q + j + 1] = w1 ? 1 : 0;
+                }
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < kBmRows / 64; ++r) {
+            const int pr = lane + 64 * r;
+            if (pr < Pc) {
+                const int p = p0 + pr;
+                if (w0) U[((int64_t)c * Q + j) * rows + p] = u0[r];
+                if (w1) U[((int64_t)c * Q + j + 1) * rows + p] = u1[r];
             }
         }
     }
@@ -204,6 +213,7 @@ extern "C" int qsft_assemble_blocks(const qsft_assemble_desc* h) {
         return -1;
     const long long rows = (long long)h->R * h->p1;
     if (h->energy && (!h->bin_major || rows > kBmRows)) return -1;   // energies need whole columns per tile
+    if (h->written && !h->energy) return -1;                          // the write test is the energy
     long long blocks;
     if (h->bin_major) {
         blocks = (long long)h->C * ((h->Q + kBmBins - 1) / kBmBins) * ((rows + kBmRows - 1) / kBmRows);

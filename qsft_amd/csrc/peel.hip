@@ -407,7 +407,8 @@ __device__ __forceinline__ void apply_pending(const qsft_peel_desc& d, int t, in
     }
 }
 
-// Write bin t's column back after its queue was applied, and clear the queue.
+// Write bin t's column back after its queue was applied, and clear the queue
+// (with a sparse U the column now exists there).
 template <int NR>
 __device__ __forceinline__ void store_col(const qsft_peel_desc& d, int t, const double2 (&u)[NR], int lane) {
     const int P = d.P;
@@ -417,7 +418,32 @@ __device__ __forceinline__ void store_col(const qsft_peel_desc& d, int t, const 
         const int p = lane + 64 * r;
         if (p < P) col[p] = gd2{u[r].x, u[r].y};
     }
-    if (lane == 0) gptr(d.pend_head)[t] = -1;                      // the queue is consumed
+    if (lane == 0) {
+        gptr(d.pend_head)[t] = -1;                                 // the queue is consumed
+        if (d.flags & QSFT_PEEL_SPARSE_U) gptr(d.written)[t] = 1;
+    }
+}
+
+// Sparse U: the column of bin t = (c, j) that the assembler did not write,
+// formed as the assembler forms it: the FP32 sample promoted to FP64 plus the
+// Philox noise of the element (same counter, key and rounding).
+template <int NR>
+__device__ __forceinline__ void rebuild_col(const qsft_peel_desc& d, int c, int j, double2 (&u)[NR], int lane) {
+    const int P = d.P;
+    const int64_t Q = d.Q;
+    const float2* __restrict__ xs = reinterpret_cast<const float2*>(d.xsrc) + (size_t)c * P * Q + j;
+    const double comp_sd = d.noise_sigma * 0.70710678118654752440;
+    const uint64_t key = *d.noise_key;
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+        const int p = lane + 64 * r;
+        u[r] = make_double2(0.0, 0.0);
+        if (p < P) {
+            const float2 v = xs[(size_t)p * Q];
+            u[r] = make_double2((double)v.x, (double)v.y);
+            if (d.noise_sigma > 0.0) u[r] = add_philox(u[r], c, p, P, Q, j, comp_sd, key);
+        }
+    }
 }
 
 // After the last pass: apply what its kernel B queued, so U ends as the
@@ -446,7 +472,8 @@ __global__ __launch_bounds__(kSlotThreads) void k_peel_flush(const qsft_peel_des
         const int t = base + __builtin_ctzll(todo);
         todo &= todo - 1;
         double2 u[kRowsPerLane];
-        load_col(u, reinterpret_cast<const double2*>(d.U) + (size_t)t * P, P, lane);
+        if ((d.flags & QSFT_PEEL_SPARSE_U) && !d.written[t]) rebuild_col(d, t / d.Q, t % d.Q, u, lane);
+        else load_col(u, reinterpret_cast<const double2*>(d.U) + (size_t)t * P, P, lane);
         const Pending pd = pending_head(d, d.pend_head[t], (int)iters - 1);
         load_krow_now(d, pd.ent, (int)iters - 1, lane, kfirst[wave]);
         apply_pending(d, t, t / d.Q, (int)iters - 1, pd, u, kfirst[wave], kws[wave], tab, mq, lane, offbuf, staged);
@@ -591,6 +618,10 @@ __global__ __launch_bounds__(kBinThreads) __attribute__((amdgpu_waves_per_eu(3))
     }
     const uint32_t scanned = (uint32_t)__popcll(todo);
     int nz = (int)nheld, ns = 0;                                   // wave-uniform
+    // sparse U: due bins whose column the assembler skipped (only this wave
+    // writes its bins' flags, in store_col, after their step)
+    const uint64_t unwritten =
+        (d.flags & QSFT_PEEL_SPARSE_U) ? __ballot(((todo >> lane) & 1) && !gptr(d.written)[base + lane]) : 0ull;
     int c = base / Q;                                              // group of the current bin
 
     // queue heads of the wave's bins (balls the previous pass peeled into them)
@@ -641,6 +672,11 @@ __global__ __launch_bounds__(kBinThreads) __attribute__((amdgpu_waves_per_eu(3))
 
         while (t >= (c + 1) * Q) ++c;
         const int j = t - c * Q;
+        // (a sparse U needs the assembler's energies, so P <= 128: NR = 2)
+        const bool rebuilt = NR == 2 && ((unwritten >> cur) & 1);  // wave-uniform
+        if constexpr (NR == 2) {
+            if (rebuilt) rebuild_col(d, c, j, u, lane);            // (its prefetched copy is not U)
+        }
         // balls queued on this bin by the previous pass
         apply_pending(d, t, c, it - 1, pd, u, pslot[wave][cslot].krow, kk, tab, mq, lane, offbuf, offs_staged);
         int mval = 0;                                              // mark to store (0: none)
@@ -648,7 +684,7 @@ __global__ __launch_bounds__(kBinThreads) __attribute__((amdgpu_waves_per_eu(3))
         double rho_x = 0.0, rho_y = 0.0;
         if (bin_energy(u) > thr) {                                 // not a zeroton (wave-uniform)
             ++nz;
-            if (pd.ent >= 0) {                                     // the column changed: refresh the LDS copy
+            if (pd.ent >= 0 || rebuilt) {                          // the column changed: refresh the LDS copy
 #pragma unroll
                 for (int r = 0; r < NR; ++r) {
                     const int p = lane + 64 * r;
@@ -747,7 +783,7 @@ __global__ __launch_bounds__(kBinThreads) __attribute__((amdgpu_waves_per_eu(3))
         wait_vm<0>();                                              // the next bin's data has landed
         __asm__ volatile("" ::: "memory");
         // this bin's stores
-        if (pd.ent >= 0) store_col(d, t, u, lane);
+        if (pd.ent >= 0 || rebuilt) store_col(d, t, u, lane);
         if (mval && lane == 0) mark[t] = mval;
         if (single) {
             if (lane < nw) reinterpret_cast<QSFT_GLOBAL uint32_t*>(kcur + (size_t)t * n_pad)[lane] = kk[lane];
@@ -1114,7 +1150,7 @@ int hcap_log2_for(long long CQ) {
     return l;
 }
 struct PeelLayout {
-    size_t off[21];
+    size_t off[23];
     size_t total;
 };
 PeelLayout peel_layout(long long CQ, int C, int n_pad, int hl2, int max_iter, int out_cap) {
@@ -1133,6 +1169,7 @@ PeelLayout peel_layout(long long CQ, int C, int n_pad, int hl2, int max_iter, in
         (size_t)CQ * 4, (size_t)CQ * 4,                                      // 16-17 mark sstamp
         (size_t)CQ * 4, (size_t)CQ * 4 * C,                                  // 18-19 pend_head pend_next
         H * 4,                                                               // 20 live
+        (size_t)CQ, 8,                                                       // 21-22 written noise_key
     };
     PeelLayout lay{};
     size_t o = 0;
@@ -1204,6 +1241,10 @@ extern "C" int qsft_peel_bind(qsft_peel_desc* h, int q, int n, int b, int C, int
     d.pend_head = reinterpret_cast<int32_t*>(base + lay.off[18]);
     d.pend_next = reinterpret_cast<int32_t*>(base + lay.off[19]);
     d.live = reinterpret_cast<uint32_t*>(base + lay.off[20]);
+    d.written = reinterpret_cast<uint8_t*>(base + lay.off[21]);
+    d.noise_key = reinterpret_cast<uint64_t*>(base + lay.off[22]);
+    d.xsrc = nullptr;
+    d.noise_sigma = 0.0;
     d.flags = 0;
     d.energy = nullptr;
     *h = d;

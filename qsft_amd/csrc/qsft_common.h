@@ -66,6 +66,7 @@ template <typename T> using cplx_t = typename cplx<T>::type;
 // increments 0x9E3779B9 / 0xBB67AE85, ten rounds.
 struct u4 { uint32_t x, y, z, w; };
 __host__ __device__ __forceinline__ u4 philox4x32_10(u4 c, uint32_t k0, uint32_t k1) {
+#pragma unroll
     for (int r = 0; r < 10; ++r) {
         const uint64_t p0 = (uint64_t)0xD2511F53u * c.x;
         const uint64_t p1 = (uint64_t)0xCD9E8D57u * c.z;
@@ -96,7 +97,9 @@ __device__ __forceinline__ double2 normal2(uint64_t counter, uint64_t seed) {
 // Four independent N(0, 1) draws from one Philox block: Box-Muller in FP32 on
 // two pairs of 24-bit uniforms, u1 in (0, 1) so the log is finite (largest
 // radius sqrt(50 ln 2) = 5.9). v_log_f32 is log2 and v_sin_f32 / v_cos_f32
-// take revolutions, so the angle needs no 2 pi scaling.
+// take revolutions, so the angle needs no 2 pi scaling; v_sqrt_f32 (about
+// 1 ulp) replaces the correctly rounded square root, whose expansion costs
+// a dozen more instructions for no visible difference in a noise draw.
 __device__ __forceinline__ float4 normal4(uint64_t counter, uint64_t seed) {
     const u4 r = philox4x32_10(u4{(uint32_t)counter, (uint32_t)(counter >> 32), 0u, 0u},
                                (uint32_t)seed, (uint32_t)(seed >> 32));
@@ -104,10 +107,28 @@ __device__ __forceinline__ float4 normal4(uint64_t counter, uint64_t seed) {
     const float u1a = ((float)(r.x >> 8) + 0.5f) * s24, u2a = (float)(r.y >> 8) * s24;
     const float u1b = ((float)(r.z >> 8) + 0.5f) * s24, u2b = (float)(r.w >> 8) * s24;
     const float m2ln2 = -1.3862943611198906f;                      // -2 ln 2
-    const float ra = __builtin_sqrtf(m2ln2 * __builtin_amdgcn_logf(u1a));
-    const float rb = __builtin_sqrtf(m2ln2 * __builtin_amdgcn_logf(u1b));
+    const float ra = __builtin_amdgcn_sqrtf(m2ln2 * __builtin_amdgcn_logf(u1a));
+    const float rb = __builtin_amdgcn_sqrtf(m2ln2 * __builtin_amdgcn_logf(u1b));
     return make_float4(ra * __builtin_amdgcn_cosf(u2a), ra * __builtin_amdgcn_sinf(u2a),
                        rb * __builtin_amdgcn_cosf(u2b), rb * __builtin_amdgcn_sinf(u2b));
+}
+
+// Transform-domain noise of the working element (c, p, j) (row p of group c,
+// bin j; `rows` rows per group, Q bins): one Philox block per bin pair
+// (2h, 2h + 1) of a row, components (x, y) for the even bin and (z, w) for the
+// odd one, each scaled by the per-component sd and added with one rounding.
+// The assembler and the peeler's rebuild of an unwritten column use exactly
+// this, so both give the same bits.
+__device__ __forceinline__ uint64_t noise_counter(int c, int p, int rows, int64_t Q, int64_t j) {
+    return ((uint64_t)c * rows + p) * (uint64_t)((Q + 1) >> 1) + (uint64_t)(j >> 1);
+}
+__device__ __forceinline__ double2 noise_add(double2 v, float zx, float zy, double comp_sd) {
+    return make_double2(fma(comp_sd, (double)zx, v.x), fma(comp_sd, (double)zy, v.y));
+}
+__device__ __forceinline__ double2 add_philox(double2 v, int c, int p, int rows, int64_t Q, int64_t j,
+                                              double comp_sd, uint64_t seed) {
+    const float4 z = normal4(noise_counter(c, p, rows, Q, j), seed);
+    return (j & 1) ? noise_add(v, z.z, z.w, comp_sd) : noise_add(v, z.x, z.y, comp_sd);
 }
 
 // Cross-lane moves for all-lane reductions, on the VALU (no LDS traffic).

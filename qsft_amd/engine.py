@@ -102,10 +102,14 @@ class Batch:
     FP32) or "direct" (per-query int8 MFMA phase + VALU w-table accumulation,
     FP32 or FP64; the plain path). fused_screen: the assembler also writes each
     bin's first-pass energy (transforms with P <= 128), so the peeler's
-    zeroton screen does not read U again."""
+    zeroton screen does not read U again. sparse_u (with those energies and
+    FP32 samples): the assembler writes only the columns above the zeroton
+    cutoff; the peeler rebuilds another column from the samples and the same
+    noise when a ball first lands on it (about a fifth of U is written at the
+    K1 shape)."""
 
     def __init__(self, specs, precision="fp32", max_iter=15, device=None, eval_mode=None, fused_screen=True,
-                 host_rows=True, tail_stream=True, own_stream=False):
+                 host_rows=True, tail_stream=True, own_stream=False, sparse_u=True):
         self.specs = list(specs)
         self.precision = precision
         if eval_mode is None:
@@ -136,19 +140,27 @@ class Batch:
             item = dict(Q=Q, C=C, R=P // (sp.n + 1), p1=sp.n + 1, src=x, src_rows=P,
                         gsel=list(range(C)), rsel=list(range(P // (sp.n + 1))), U=U, bin_major=True,
                         energy=energy)
-            if sp.noise_sd > 0:
-                item.update(sigma=float(sp.noise_sd / np.sqrt(Q)), seed=sp.seed)
+            sigma = float(sp.noise_sd / np.sqrt(Q)) if sp.noise_sd > 0 else 0.0
+            if sigma > 0:
+                item.update(sigma=sigma, seed=sp.seed)
             asm.append(item)
             pl.append(dict(q=sp.q, n=sp.n, b=sp.b, C=C, P=P, p1=sp.n + 1,
                            method=L.QSFT_DETECT_NSO1 if sp.method == "nso" else L.QSFT_DETECT_NOISELESS,
                            cutoff=sp.threshold(), Mt=Mt, D=Dd, U=U, shared_d=True, energy=energy))
+            if sparse_u and energy is not None and precision == "fp32":
+                pl[-1]["sparse"] = dict(x=x, sigma=sigma)
         self.eval = K.EvalMfma(ev) if eval_mode == "mfma" else K.EvalCosets(ev, code)
         self.dft = K.Dft(df, code)
-        self.asm = K.Assemble(asm)
         # packed rows: at most a few per planted support entry (noise adds some), kept
         # in pinned host memory that the pack kernels write directly
         cap = sum(4 * sp.S + 1024 for sp in self.specs) if host_rows else None
         self.peel = K.PeelBatch(pl, max_iter=max_iter, host_rows=host_rows, cap_rows=cap, final_u=False)
+        for i, it in enumerate(pl):
+            if "sparse" in it:
+                written, key = self.peel.sparse_targets(i)
+                asm[i].update(written=written, cutoff=it["cutoff"], seed_out=key)
+        self.asm_items = asm
+        self.asm = K.Assemble(asm)
         # the peeling tail (last queued balls, output table, pack into host
         # memory) runs on a second stream, overlapping the next batch's work;
         # the next launch of this batch waits for it

@@ -320,7 +320,10 @@ def dft(items, precision):
 class Assemble:
     """items: dicts Q, C, R, p1, src (tensor), src_rows, gsel, rsel, U (out), and
     noise (tensor, complex128 [C][R*p1][Q]) or sigma+seed (Philox) or neither;
-    optional energy (float64 [C][Q] out: per-bin |column|^2, bin-major only)."""
+    optional energy (float64 [C][Q] out: per-bin |column|^2, bin-major only);
+    optional written + cutoff + seed_out (device addresses from a sparse
+    PeelBatch: PeelBatch.sparse_targets): only the columns above the zeroton
+    cutoff are written."""
 
     def __init__(self, items):
         descs, blocks = [], []
@@ -348,6 +351,10 @@ class Assemble:
             d.U = _ptr(it["U"])
             if it.get("energy") is not None:
                 d.energy = _ptr(it["energy"])
+            if it.get("written"):                                  # sparse U for the peeler
+                d.written = int(it["written"])
+                d.cutoff = float(it["cutoff"])
+                d.seed_out = int(it["seed_out"])
             nb = lib.qsft_assemble_blocks(ctypes.byref(d))
             if nb < 0:
                 raise ValueError("bad assemble shape (energies need bin-major U with R*p1 <= 128)")
@@ -398,9 +405,11 @@ class PeelBatch:
     items: dicts q, n, b, C, P, p1, method, cutoff, Mt, D, U (complex128,
     bin-major [C][Q][P]), optional shared_d (all groups use the delay rows of
     group 0; compared on the device when absent), unit_d (+1 / -1: delay rows
-    are block offsets + s e_i, 0: arbitrary rows; detected when absent) and
+    are block offsets + s e_i, 0: arbitrary rows; detected when absent),
     energy (float64 [C][Q] first-pass column energies written by Assemble into
-    the same U).
+    the same U) and sparse (dict x = the assembler's FP32 samples [C][P][Q],
+    sigma = its Philox sigma or 0): U then holds only the columns the
+    assembler wrote (see sparse_targets).
     """
 
     def __init__(self, items, max_iter=15, host_rows=False, cap_rows=None, final_u=True):
@@ -440,6 +449,15 @@ class PeelBatch:
                 d.flags |= L.QSFT_PEEL_UNIT_D | (L.QSFT_PEEL_UNIT_D_NEG if unit < 0 else 0)
             if it.get("energy") is not None:
                 d.energy = _ptr(it["energy"])
+            sp = it.get("sparse")
+            if sp is not None:
+                x = sp["x"]
+                if (it.get("energy") is None or x.dtype != torch.complex64
+                        or tuple(x.shape) != (C, P, it["U"].shape[1])):
+                    raise ValueError("a sparse U needs the assembler's energies and FP32 samples [C][P][Q]")
+                d.xsrc = _ptr(x)
+                d.noise_sigma = float(sp.get("sigma", 0.0))
+                d.flags |= L.QSFT_PEEL_SPARSE_U
             self.descs.append(d)
             self.ws.append(ws)
         self.bins = [lib.qsft_peel_bin_blocks(ctypes.byref(d)) for d in self.descs]
@@ -457,6 +475,10 @@ class PeelBatch:
         else:
             self.packed = torch.empty(size, dtype=torch.uint8, device=dev)
             self.hdr_host = torch.empty(self.hdr_bytes, dtype=torch.uint8).pin_memory()
+
+    def sparse_targets(self, i):
+        """(written, noise_key) device addresses of instance i, for its assembler."""
+        return self.descs[i].written, self.descs[i].noise_key
 
     def run(self, host_loop=False, tail_stream=None, tail_event=None, passes=None):
         """The peeling passes on the current stream, then the tail (last

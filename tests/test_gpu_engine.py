@@ -8,6 +8,7 @@ pytestmark = pytest.mark.gpu
 
 if torch.cuda.is_available():
     from qsft_amd import engine as E
+    from qsft_amd import kernels as K
 
 
 def _same(a, b, rtol):
@@ -80,10 +81,14 @@ def test_fused_screen_matches_streaming_screen():
         _same((a[0], a[1]), (b[0], b[1]), rtol=1e-12)
 
 
-def test_assembler_energies_are_column_norms():
+@pytest.mark.parametrize("sparse_u", [False, True])
+def test_assembler_energies_are_column_norms(sparse_u):
+    """Energies are the norms of the columns; a sparse U holds exactly the
+    columns above the zeroton cutoff (the others keep what was there)."""
     rng = np.random.default_rng(9)
     sp = E.random_spec(4, 30, 200, 4, 3, 2, 12.0, rng=rng)
-    bt = E.Batch([sp], precision="fp32")
+    bt = E.Batch([sp], precision="fp32", sparse_u=sparse_u)
+    bt.bufs[0]["U"].fill_(float("nan"))
     bt.eval.launch()                                                     # up to the assembler only:
     bt.dft.launch()                                                      # the peeler changes U
     bt.asm.launch(4)
@@ -91,7 +96,14 @@ def test_assembler_energies_are_column_norms():
     U = bt.bufs[0]["U"].cpu().numpy()                                   # [C][Q][P]
     e = bt.bufs[0]["energy"].cpu().numpy()                              # [C][Q]
     ref = np.linalg.norm(U, axis=2) ** 2
-    np.testing.assert_allclose(e, ref, rtol=1e-13, atol=0)
+    if not sparse_u:
+        np.testing.assert_allclose(e, ref, rtol=1e-13, atol=0)
+        return
+    nz = e > sp.threshold() * sp.P
+    assert 0 < int(nz.sum()) < nz.size
+    np.testing.assert_allclose(e[nz], ref[nz], rtol=1e-13, atol=0)
+    assert np.all(np.isnan(U[~nz]))                                     # skipped columns untouched
+    assert np.array_equal(_written_flags(bt, 0).reshape(nz.shape), nz.astype(np.uint8))
 
 
 def test_host_rows_match_device_copy():
@@ -125,3 +137,48 @@ def test_short_pass_launch_is_finished_on_fetch():
         assert np.array_equal(ka, kb) and np.array_equal(oa, ob)
         assert np.array_equal(va, vb)
     assert short.passes <= short.max_iter and short.peel.last_iters >= 2
+
+
+def _written_flags(bt, i):
+    d, ws = bt.peel.descs[i], bt.peel.ws[i]
+    off = d.written - ws.data_ptr()
+    sp = bt.specs[i]
+    return ws[off: off + sp.C * sp.Q].cpu().numpy()
+
+
+def _identical(a, b):
+    assert [tuple(r) for r in a[0]] == [tuple(r) for r in b[0]]      # same rows, same order
+    assert np.array_equal(np.asarray(a[1]), np.asarray(b[1]))         # same bits
+
+
+@pytest.mark.parametrize("everything", [False, True])
+def test_sparse_u_matches_dense_u(everything):
+    """Sparse U: the assembler writes only the columns above the zeroton
+    cutoff, the peeler rebuilds any other column it reads from the samples and
+    the same Philox noise. Bit-identical rows to a fully written U. With
+    `everything` the assembler writes no column at all (its cutoff is +inf),
+    so every column the peeler reads, from the first pass on, is a rebuilt one."""
+    rng = np.random.default_rng(12)
+    specs = [E.random_spec(4, 50, 1000, 6, 3, 2, snr, rng=rng) for snr in (3.0, 10.0)]
+    specs += [E.random_spec(4, 30, 100, 4, 3, 2, None, rng=rng), E.random_spec(3, 20, 90, 4, 3, 2, 20.0, rng=rng)]
+    print("sparse U vs dense U, everything rebuilt:", everything, flush=True)
+    sparse = E.Batch(specs, precision="fp32", sparse_u=True)
+    dense = E.Batch(specs, precision="fp32", sparse_u=False)
+    assert all(d.flags & 8 for d in sparse.peel.descs) and not any(d.flags & 8 for d in dense.peel.descs)
+    if everything:
+        sparse.asm = K.Assemble([dict(it, cutoff=float("inf")) for it in sparse.asm_items])
+    sparse.launch(seed_offset=5)
+    dense.launch(seed_offset=5)
+    for a, b in zip(sparse.results(), dense.results()):
+        _identical(a, b)
+    for i, sp in enumerate(specs):
+        written = _written_flags(sparse, i)
+        energy = sparse.bufs[i]["energy"].cpu().numpy().ravel()
+        thr = sp.threshold() * sp.P
+        rebuilt = int(np.sum((written == 1) & (energy <= thr)))
+        if everything:
+            # every first-pass non-zeroton was read (and so rebuilt and written back)
+            assert np.all(written[energy > thr] == 1), i
+        else:
+            assert np.array_equal(written[energy > thr], np.ones(int(np.sum(energy > thr)), np.uint8)), i
+        print(f"instance {i}: written {int(written.sum())} of {written.size}, rebuilt below the cutoff {rebuilt}")
