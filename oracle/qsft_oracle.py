@@ -25,7 +25,9 @@ How each part is pinned (tests/test_oracle.py):
     instances, hand-built bins that exercise each decision rule (zeroton,
     bin-mismatch multiton, residual multiton, Jacobi scan, last-writer rho,
     append-order running mean, stop rule, 15-iteration cap), and dense
-    brute-force spectra for q^n <= 2^16.
+    brute-force spectra for q^n <= 2^16;
+  * held-out NMSE: a hand-computed q = 2 case, exact predictions (NMSE 0),
+    doubled coefficients (NMSE 1) and the empty spectrum.
 """
 from __future__ import annotations
 
@@ -480,6 +482,27 @@ def nmse(gwht_est, signal_w):
     num = np.sum(np.abs(np.array(list(diff.values()))) ** 2)
     den = np.sum(np.abs(np.array(list(signal_w.values()))) ** 2)
     return float(num / den)
+
+
+def heldout_nmse(sample_idx_dec, samples, beta, q, n, batch=10000):
+    """Held-out NMSE of a recovered spectrum (qsft/test_helper.py:235-260,
+    TestHelper._test_qary): predict every test sample as
+    sum_k beta[k] exp(2 pi i <m, k> / q) with the unreduced integer phase, in
+    batches of 10000 queries, and return ||y_hat - y||^2 / ||y||^2; an empty
+    beta gives 1."""
+    if len(beta.keys()) == 0:
+        return 1
+    keys = np.array(list(beta.keys()))
+    vals = np.array(list(beta.values()))
+    idx = list(sample_idx_dec)
+    y_hat = []
+    for i in range(0, len(idx), batch):
+        m = dec_to_qary_vec(idx[i:i + batch], q, n)
+        freqs = np.array(m).T @ keys.T
+        y_hat.append(np.exp(2j * np.pi * freqs / q) @ vals)
+    y_hat = np.concatenate(y_hat)
+    samples = np.asarray(list(samples))
+    return np.linalg.norm(y_hat - samples) ** 2 / np.linalg.norm(samples) ** 2
 
 
 def cpu_count():

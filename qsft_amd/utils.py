@@ -4,7 +4,22 @@ These are the small integer/ordering utilities of the public API; the
 numeric work of the transform runs in the HIP library. `gwht` here runs the
 device DFT kernel.
 """
+import json
+
 import numpy as np
+
+
+class NpEncoder(json.JSONEncoder):
+    """JSON encoder for numpy scalars and arrays (qsft/utils.py:199-207)."""
+
+    def default(self, obj):
+        if isinstance(obj, np.integer):
+            return int(obj)
+        if isinstance(obj, np.floating):
+            return float(obj)
+        if isinstance(obj, np.ndarray):
+            return obj.tolist()
+        return super().default(obj)
 
 
 def qary_ints(m, q, dtype=int):

@@ -327,3 +327,39 @@ def test_unit_delay_signatures_match_row_products(channel):
     assert_same_result(got, ref, rtol=0.0)
     assert info["trace"] == info0["trace"]
     np.testing.assert_array_equal(U1, U0)
+
+
+def test_synthetic_helper_heldout_nmse(tmp_path):
+    """TestHelper / SyntheticHelper (qsft/test_helper.py, synt_exp/synt_src/
+    synthetic_helper.py): decode the training signal, score the spectrum on
+    noiseless uniform held-out queries with the GPU evaluator. Equal to the
+    oracle's _test_qary on the same queries: ~0 without noise, 1e-9 relative
+    with noise; doubled coefficients score 1, an empty spectrum 1."""
+    from qsft_amd.synthetic_helper import SyntheticHelper
+    from qsft_amd.synthetic_signal import generate_signal_w
+
+    q, n, S, b = 4, 20, 40, 4
+    np.random.seed(11)
+    signal_w, locq, strengths = generate_signal_w(n, q, S, 1, 10, full=False)
+    signal_args = dict(n=n, q=q, locq=locq, strengths=strengths, noise_sd=0.0, signal_w=signal_w)
+    sub_args = dict(num_subsample=3, num_repeat=2, b=b)
+    helper = SyntheticHelper(signal_args, ["qsft"], sub_args, dict(n_samples=4000), tmp_path)
+    assert (tmp_path / "config.json").is_file() and (tmp_path / "test").is_dir()
+    test_t = helper.test_signal.signal_t
+    idx, samples = list(test_t.keys()), list(test_t.values())
+    assert len(idx) == 4000
+    for noise_sd, tol in ((0.0, None), (3.0, 1e-9)):
+        out = helper.compute_model("qsft", dict(sub_args, noise_sd=noise_sd), report=True)
+        beta = out["gwht"]
+        got = helper.test_model("qsft", beta=beta)
+        ref = O.heldout_nmse(idx, samples, beta, q, n)
+        print(f"noise_sd {noise_sd}: found {len(beta)}, held-out NMSE {got:.3e} (oracle {ref:.3e})", flush=True)
+        if tol is None:
+            assert got < 1e-20 and ref < 1e-20
+            exact = beta
+        else:
+            assert 1e-8 < ref < 1 and abs(got - ref) <= tol * ref
+    assert helper.test_model("qsft", beta={k: 2 * v for k, v in exact.items()}) == pytest.approx(1.0, rel=1e-9)
+    assert helper.test_model("qsft", beta={}) == 1
+    with pytest.raises(NotImplementedError):
+        helper.compute_model("lasso", {})

@@ -385,3 +385,33 @@ def test_dense_spectrum_brute_force():
     gw = O.qsft_transform(sig, 3, 1, b, "identity", "identity")
     for k, v in gw.items():
         assert abs(spec[k] - v) < 1e-9
+
+
+# ---------------------------------------------------------- held-out NMSE
+
+
+def test_heldout_nmse_hand_case():
+    """q = 2, n = 1: beta = {0: 1, 1: 1} predicts y(0) = 2, y(1) = 0; against
+    samples (2, 1) the error is (0, -1): NMSE = 1 / 5."""
+    assert O.heldout_nmse([0, 1], [2.0, 1.0], {(0,): 1.0, (1,): 1.0}, 2, 1) == pytest.approx(0.2, abs=1e-15)
+    assert O.heldout_nmse([1, 0], [0.0, 2.0], {(0,): 1.0, (1,): 1.0}, 2, 1) == pytest.approx(0.0, abs=1e-30)
+
+
+@pytest.mark.parametrize("q,n,S", [(4, 12, 30), (3, 9, 20), (20, 4, 15)])
+def test_heldout_nmse_exact_and_doubled(q, n, S):
+    """Predictions from the true spectrum give NMSE ~0; doubling every
+    coefficient gives exactly 1 (y_hat = 2 y); the empty spectrum gives 1.
+    Samples come from a brute-force sum with phases reduced mod q."""
+    rng = np.random.default_rng(q + n)
+    locq = rng.integers(0, q, size=(n, S))
+    a = rng.uniform(1, 10, S) * np.exp(2j * np.pi * rng.uniform(size=S))
+    beta = {}
+    for s in range(S):
+        beta[tuple(locq[:, s])] = beta.get(tuple(locq[:, s]), 0) + a[s]
+    idx = [int(v) for v in rng.integers(0, q ** n, size=2500)]
+    dig = O.dec_to_qary_vec(idx, q, n)
+    w = np.exp(2j * np.pi * np.arange(q) / q)
+    y = np.array([sum(v * w[int(np.dot(dig[:, i], k)) % q] for k, v in beta.items()) for i in range(len(idx))])
+    assert O.heldout_nmse(idx, y, beta, q, n, batch=1000) < 1e-24
+    assert O.heldout_nmse(idx, y, {k: 2 * v for k, v in beta.items()}, q, n) == pytest.approx(1.0, rel=1e-12)
+    assert O.heldout_nmse(idx, y, {}, q, n) == 1
