@@ -182,3 +182,91 @@ def test_sparse_u_matches_dense_u(everything):
         else:
             assert np.array_equal(written[energy > thr], np.ones(int(np.sum(energy > thr)), np.uint8)), i
         print(f"instance {i}: written {int(written.sum())} of {written.size}, rebuilt below the cutoff {rebuilt}")
+
+
+def test_k2_served_fp32_matches_fp64_path():
+    """q = 20, n = 16, S = 2000, b = 3 (8000 bins, radix-20 DFT), NSO R = 2, 20 dB:
+    the served FP32 path (matrix-core evaluation, FP32 DFT, FP64 peel) against
+    the FP64 path on the same Philox noise: the same NMSE to the planted
+    spectrum within 2 %, supports differing in at most 0.5 % of the keys and
+    shared coefficients within 1e-4 relative."""
+    rng = np.random.default_rng(20)
+    sp = E.random_spec(20, 16, 2000, 3, 3, 2, 20.0, rng=rng)
+    print("K2 fp32 vs fp64", flush=True)
+    f32, f64 = E.Batch([sp], precision="fp32"), E.Batch([sp], precision="fp64")
+    f32.launch(seed_offset=3)
+    f64.launch(seed_offset=3)
+    (k32, v32, _, _), = f32.results()
+    (k64, v64, _, _), = f64.results()
+    n32, n64 = E.nmse(k32, v32, sp), E.nmse(k64, v64, sp)
+    d32 = {tuple(r): v for r, v in zip(k32, v32)}
+    d64 = {tuple(r): v for r, v in zip(k64, v64)}
+    common = set(d32) & set(d64)
+    diff = len(set(d32) ^ set(d64))
+    rel = max(abs(d32[k] - d64[k]) / abs(d64[k]) for k in common)
+    print(f"NMSE fp32 {n32:.4e} fp64 {n64:.4e}; keys {len(d32)} / {len(d64)}, differing {diff}, max rel {rel:.2e}")
+    assert abs(n32 - n64) <= 0.02 * n64 + 1e-12
+    assert diff <= 0.005 * len(d64)
+    assert rel < 1e-4
+
+
+def test_k4_large_job_noiseless_support_exact():
+    """The large job of the serving mix: q = 4, n = 100, S = 5000, b = 7 (16384
+    bins), NSO R = 2 (P = 202 rows: the four-rows-per-lane peeler, two 64-digit
+    blocks in the evaluator). Noiseless through the served FP32 engine: the
+    support is exact and the values match to FP32 level."""
+    rng = np.random.default_rng(41)
+    sp = E.random_spec(4, 100, 5000, 7, 3, 2, None, rng=rng)
+    print("K4 large job, noiseless", flush=True)
+    bt = E.Batch([sp], precision="fp32")
+    bt.launch()
+    (k, v, _, _), = bt.results()
+    assert {tuple(r) for r in k} == {tuple(c) for c in sp.locq.T}
+    assert E.nmse(k, v, sp) < 1e-10
+
+
+def _k3_mix(count, rng):
+    """Uneven instances by the K3 recipe: q in {2, 3, 4, 20}, n in [8, 64], S
+    log-uniform in [10, 2000] (at most q^n / 8), b = ceil(log_q S) + 1, C = 3,
+    NSO R = 2, SNR uniform in [5, 30] dB."""
+    specs = []
+    for _ in range(count):
+        q = int(rng.choice([2, 3, 4, 20]))
+        n = int(rng.integers(8, 65))
+        S = int(np.exp(rng.uniform(np.log(10), np.log(2000))))
+        S = max(2, min(S, q ** n // 8))
+        b = int(np.ceil(np.log(S) / np.log(q))) + 1
+        specs.append(E.random_spec(q, n, S, b, 3, 2, float(rng.uniform(5, 30)), rng=rng))
+    return specs
+
+
+def test_k3_uneven_mix_batch_equals_alone():
+    """64 instances of the K3 mix in one set of launches: every instance gives
+    the rows it gives alone (a sample of 10). Noiseless copies of 16 of the
+    shapes recover only true keys, with their exact values, and 95 % of all
+    planted keys (peeling with C = 3 groups is not certain to finish every
+    instance)."""
+    rng = np.random.default_rng(33)
+    specs = _k3_mix(64, rng)
+    print("K3 mix:", sorted({(s.q, s.b) for s in specs}), flush=True)
+    mix = E.Batch(specs, precision="fp32")
+    mix.launch(seed_offset=9)
+    got = mix.results()
+    for i in rng.choice(len(specs), size=10, replace=False):
+        alone = E.Batch([specs[i]], precision="fp32")
+        alone.launch(seed_offset=9)
+        (k, v, _, _), = alone.results()
+        _same((got[i][0], got[i][1]), (k, v), rtol=1e-12)
+    clean = [E.TransformSpec(**{**s.__dict__, "noise_sd": 0.0}) for s in specs[:16]]
+    bt = E.Batch(clean, precision="fp32")
+    bt.launch()
+    found = planted = 0
+    for (k, v, _, _), sp in zip(bt.results(), clean):
+        truth = {tuple(c): a for c, a in zip(sp.locq.T, sp.strengths)}
+        for r, val in zip(k, v):
+            assert tuple(r) in truth, (sp.q, sp.n, sp.S, sp.b)
+            assert abs(val - truth[tuple(r)]) <= 1e-4 * abs(truth[tuple(r)])
+        found += len(k)
+        planted += sp.S
+    print(f"noiseless K3 copies: {found} of {planted} keys", flush=True)
+    assert found >= 0.95 * planted
