@@ -44,6 +44,7 @@ namespace qsft {
 typedef int v4i __attribute__((ext_vector_type(4)));
 typedef unsigned int v4u __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
+typedef float v2f __attribute__((ext_vector_type(2)));
 typedef _Float16 v8h __attribute__((ext_vector_type(8)));
 
 constexpr int kMfThreads = 256;
@@ -446,13 +447,17 @@ __device__ __forceinline__ void eval_mfma_body(const qsft_eval_mfma_desc* __rest
     }
 
     // store: lane holds column n = r16 -> delay row p = 8 (pt0 + pt) + n / 2, Re (n even) / Im (n odd),
-    // for base queries l = lbase + 16 t + 4 g + v. Pair Re/Im across lanes (xor 1).
+    // for base queries l = lbase + 16 t + 4 g + v. Pair Re/Im across lanes (xor 1); the even lane
+    // writes queries l0, l0 + 1 and the odd lane l0 + 2, l0 + 3, each pair as one 16-byte store
+    // (l0 is a multiple of 4; Q is a power of q >= 2, so an odd Q is only the q^b < 4 case that
+    // the per-query branch handles).
     float2* __restrict__ x = reinterpret_cast<float2*>(d.x) + (size_t)c * P * Q;
     const bool odd = r16 & 1;
+    const bool even_q = (Q & 1) == 0;
     const float unscale = ldexpf(1.0f, -d.bexp);                  // exact: 2^-bexp is a normal float
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
-        const int l0 = lbase + 16 * t + 4 * g;
+        const int l0 = lbase + 16 * t + 4 * g + (odd ? 2 : 0);
 #pragma unroll
         for (int pt = 0; pt < NPT; ++pt) {
             if (pt < npt_here) {
@@ -461,14 +466,15 @@ __device__ __forceinline__ void eval_mfma_body(const qsft_eval_mfma_desc* __rest
 #pragma unroll
                 for (int v = 0; v < 4; ++v) o[v] = __shfl_xor(a[v], 1);
                 const int p = 8 * (pt0 + pt) + (r16 >> 1);
-                if (p < P) {
-                    float2* row = x + (size_t)p * Q;
-                    if (!odd) {
-                        if (l0 < Q) row[l0] = make_float2(a[0] * unscale, o[0] * unscale);
-                        if (l0 + 1 < Q) row[l0 + 1] = make_float2(a[1] * unscale, o[1] * unscale);
+                if (p < P && l0 < Q) {
+                    QSFT_GLOBAL float* row = gptr(reinterpret_cast<float*>(x + (size_t)p * Q + l0));
+                    const v4f v = odd ? v4f{o[2] * unscale, a[2] * unscale, o[3] * unscale, a[3] * unscale}
+                                      : v4f{a[0] * unscale, o[0] * unscale, a[1] * unscale, o[1] * unscale};
+                    if (even_q) {                                  // l0 even, so l0 + 1 < Q
+                        *reinterpret_cast<QSFT_GLOBAL v4f*>(row) = v;
                     } else {
-                        if (l0 + 2 < Q) row[l0 + 2] = make_float2(o[2] * unscale, a[2] * unscale);
-                        if (l0 + 3 < Q) row[l0 + 3] = make_float2(o[3] * unscale, a[3] * unscale);
+                        *reinterpret_cast<QSFT_GLOBAL v2f*>(row) = v2f{v[0], v[1]};
+                        if (l0 + 1 < Q) *reinterpret_cast<QSFT_GLOBAL v2f*>(row + 2) = v2f{v[2], v[3]};
                     }
                 }
             }
