@@ -29,9 +29,10 @@
 //   * accumulation is FP32 in the MFMA; x is scaled back by 2^-bexp (exact)
 //     and written as complex FP32.
 //
-// A workgroup owns 256 base queries (4 waves x 4 subtiles of 16) x 8 p-tiles
-// (64 delay rows) of one group. Per 16-support block it stages the 8 x 2 B
-// fragments (16 KiB) in LDS and each wave reuses every fragment for its 4
+// A workgroup owns 256 base queries (4 waves x 4 subtiles of 16) x up to 8
+// p-tiles (64 delay rows) of one group (5..8 for q in {2, 4} with n <= 64, split
+// evenly over the group's tiles). Per 16-support block it stages the p-tiles' x 2
+// B fragments (up to 16 KiB) in LDS and each wave reuses every fragment for its 4
 // subtiles; the fragments of the next three blocks are loaded straight into
 // LDS (a four-slot ring) while the current block's MFMAs run. Logical blocks are
 // dealt out XCD by XCD so the workgroups that share B fragments share an L2.
