@@ -297,7 +297,8 @@ typedef struct qsft_peel_desc {
                                  * and the identity delays (s = +1); signatures then take *
                                  * R dot products per key instead of P                   */
 #define QSFT_PEEL_UNIT_D_NEG 4  /* with QSFT_PEEL_UNIT_D: s = -1                          */
-#define QSFT_PEEL_SPARSE_U 8    /* U holds only the columns the assembler wrote (above)  */
+#define QSFT_PEEL_SPARSE_U 8    /* U holds only the columns the assembler wrote (above;  *
+                                 * ignored unless xsrc, written and noise_key are set)   */
 
 /* counter layout (uint32 words); per iteration i at 8 i + :
  *   0 singletons, 1 multitons, 2 distinct balls, 3 active flag,

@@ -407,6 +407,12 @@ __device__ __forceinline__ void apply_pending(const qsft_peel_desc& d, int t, in
     }
 }
 
+// Sparse working transform in use: the flag plus the buffers it needs (a
+// descriptor with the flag but without them is treated as a dense U).
+__device__ __forceinline__ bool sparse_u(const qsft_peel_desc& d) {
+    return (d.flags & QSFT_PEEL_SPARSE_U) && d.xsrc && d.written && d.noise_key;
+}
+
 // Write bin t's column back after its queue was applied, and clear the queue
 // (with a sparse U the column now exists there).
 template <int NR>
@@ -420,7 +426,7 @@ __device__ __forceinline__ void store_col(const qsft_peel_desc& d, int t, const 
     }
     if (lane == 0) {
         gptr(d.pend_head)[t] = -1;                                 // the queue is consumed
-        if (d.flags & QSFT_PEEL_SPARSE_U) gptr(d.written)[t] = 1;
+        if (sparse_u(d)) gptr(d.written)[t] = 1;
     }
 }
 
@@ -472,7 +478,7 @@ __global__ __launch_bounds__(kSlotThreads) void k_peel_flush(const qsft_peel_des
         const int t = base + __builtin_ctzll(todo);
         todo &= todo - 1;
         double2 u[kRowsPerLane];
-        if ((d.flags & QSFT_PEEL_SPARSE_U) && !d.written[t]) rebuild_col(d, t / d.Q, t % d.Q, u, lane);
+        if (sparse_u(d) && !d.written[t]) rebuild_col(d, t / d.Q, t % d.Q, u, lane);
         else load_col(u, reinterpret_cast<const double2*>(d.U) + (size_t)t * P, P, lane);
         const Pending pd = pending_head(d, d.pend_head[t], (int)iters - 1);
         load_krow_now(d, pd.ent, (int)iters - 1, lane, kfirst[wave]);
@@ -621,7 +627,7 @@ __global__ __launch_bounds__(kBinThreads) __attribute__((amdgpu_waves_per_eu(3))
     // sparse U: due bins whose column the assembler skipped (only this wave
     // writes its bins' flags, in store_col, after their step)
     const uint64_t unwritten =
-        (d.flags & QSFT_PEEL_SPARSE_U) ? __ballot(((todo >> lane) & 1) && !gptr(d.written)[base + lane]) : 0ull;
+        sparse_u(d) ? __ballot(((todo >> lane) & 1) && !gptr(d.written)[base + lane]) : 0ull;
     int c = base / Q;                                              // group of the current bin
 
     // queue heads of the wave's bins (balls the previous pass peeled into them)
