@@ -688,7 +688,8 @@ __global__ __launch_bounds__(kBinThreads) __attribute__((amdgpu_waves_per_eu(3))
         int mval = 0;                                              // mark to store (0: none)
         bool single = false;
         double rho_x = 0.0, rho_y = 0.0;
-        if (bin_energy(u) > thr) {                                 // not a zeroton (wave-uniform)
+        const double energy = bin_energy(u);
+        if (energy > thr) {                                        // not a zeroton (wave-uniform)
             ++nz;
             if (pd.ent >= 0 || rebuilt) {                          // the column changed: refresh the LDS copy
 #pragma unroll
@@ -749,22 +750,10 @@ __global__ __launch_bounds__(kBinThreads) __attribute__((amdgpu_waves_per_eu(3))
             ri = wave_sum(ri);
             rho_x = __ddiv_rn(rr, (double)P);
             rho_y = __ddiv_rn(ri, (double)P);
-            double qre = 0.0, qim = 0.0;
-#pragma unroll
-            for (int r = 0; r < NR; ++r) {
-                const int p = lane + 64 * r;
-                if (p < P) {
-                    const double sx = __dsub_rn(__dmul_rn(rho_x, w[r].x), __dmul_rn(rho_y, w[r].y));
-                    const double sy = __dadd_rn(__dmul_rn(rho_x, w[r].y), __dmul_rn(rho_y, w[r].x));
-                    const double ex = __dsub_rn(u[r].x, sx), ey = __dsub_rn(u[r].y, sy);
-                    qre += ex * ex;
-                    qim += ey * ey;
-                }
-            }
-            qre = wave_sum(qre);
-            qim = wave_sum(qim);
-            double res = sqrt(qre + qim);
-            res = res * res;
+            // residual energy |col - rho sig|^2 = |col|^2 - P |rho|^2 (|sig_p| = 1 and
+            // sig^H col = P rho); rounding error ~1e-16 |col|^2, far below any cutoff
+            // that separates a singleton from a multiton
+            const double res = energy - (double)P * (rho_x * rho_x + rho_y * rho_y);
 
             // --- bin match: the bin of k in group c, sum over digits dg (lane dg)
             // of (<M_c^T row dg, k> mod q) q^(b - 1 - dg), equals j
