@@ -592,6 +592,7 @@ __global__ __launch_bounds__(kBinThreads) __attribute__((amdgpu_waves_per_eu(3))
     int8_t* kb8 = reinterpret_cast<int8_t*>(kk);
     const ModQ mq((uint32_t)q);
     const double thr = d.cutoff * (double)P;
+    const double inv_p = 1.0 / (double)P;
     const double2* __restrict__ Ub = reinterpret_cast<const double2*>(d.U);
     QSFT_GLOBAL int8_t* __restrict__ kcur = gptr(d.kbin) + (size_t)(it & 1) * CQ * n_pad;   // this pass's k and rho
     QSFT_GLOBAL double* __restrict__ rcur = gptr(d.rho_bin) + (size_t)(it & 1) * CQ * 2;
@@ -748,8 +749,8 @@ __global__ __launch_bounds__(kBinThreads) __attribute__((amdgpu_waves_per_eu(3))
             }
             rr = wave_sum(rr);
             ri = wave_sum(ri);
-            rho_x = __ddiv_rn(rr, (double)P);
-            rho_y = __ddiv_rn(ri, (double)P);
+            rho_x = rr * inv_p;                                    // within 1 ulp of rr / P
+            rho_y = ri * inv_p;
             // residual energy |col - rho sig|^2 = |col|^2 - P |rho|^2 (|sig_p| = 1 and
             // sig^H col = P rho); rounding error ~1e-16 |col|^2, far below any cutoff
             // that separates a singleton from a multiton
