@@ -522,9 +522,11 @@ __device__ __forceinline__ int fast_sector(double zr, double zi, int q) {
 // nso1 digit of coordinate i (reference: the nso1 branch of
 // singleton_detection, qsft/reconstruct.py:12-31): the phase of the mean over
 // the R repeats of u_{r,0} conj(u_{r,i}), rounded to the nearest 2 pi t / q.
-__device__ __forceinline__ uint32_t nso1_digit(double zr, double zi, int q, int R) {
-    const int f = fast_sector(zr, zi, q);
-    if (f >= 0) return (uint32_t)f;
+// The exact FP64 rule, for phases the FP32 sector test leaves open (within
+// about 1e-4 of a sector boundary): kept out of line, so that its registers
+// are not reserved across the detection loop (its spills run only when it
+// is called).
+__device__ __noinline__ uint32_t nso1_slow(double zr, double zi, int q, int R) {
     zr = __ddiv_rn(zr, (double)R);
     zi = __ddiv_rn(zi, (double)R);
     const double twopi = 6.283185307179586;
@@ -539,17 +541,17 @@ __device__ __forceinline__ uint32_t nso1_digit(double zr, double zi, int q, int 
     return (uint32_t)(best % q);
 }
 
+__device__ __forceinline__ uint32_t nso1_digit(double zr, double zi, int q, int R) {
+    const int f = fast_sector(zr, zi, q);
+    return f >= 0 ? (uint32_t)f : nso1_slow(zr, zi, q, R);
+}
+
 // Noiseless digit of coordinate i (reference: qsft/reconstruct.py:100-113):
-// round(q (angle(u_i) - angle(u_0)) / (2 pi)) mod q. The phase difference is
-// the phase of u_i conj(u_0), exact to ~1e-16 rad when both magnitudes are
-// far from the FP64 range ends.
-__device__ __forceinline__ uint32_t noiseless_digit(double2 v0, double2 vi, int q) {
-    const double m0 = fmax(fabs(v0.x), fabs(v0.y)), mi = fmax(fabs(vi.x), fabs(vi.y));
-    if (m0 > 1.0e-150 && m0 < 1.0e150 && mi > 1.0e-150 && mi < 1.0e150) {
-        const double wr = vi.x * v0.x + vi.y * v0.y, wi = vi.y * v0.x - vi.x * v0.y;
-        const int f = fast_sector(wr, wi, q);
-        if (f >= 0) return (uint32_t)f;
-    }
+// round(q (angle(u_i) - angle(u_0)) / (2 pi)) mod q, from the two FP64 angles
+// (out of line, as nso1_slow) unless the FP32 sector test of the phase of
+// u_i conj(u_0), exact to ~1e-16 rad when both magnitudes are far from the
+// FP64 range ends, decides it.
+__device__ __noinline__ uint32_t noiseless_slow(double2 v0, double2 vi, int q) {
     const double twopi = 6.283185307179586;
     const double a0 = atan2(v0.y, v0.x), ai = atan2(vi.y, vi.x);
     const double v = __ddiv_rn(__dmul_rn((double)q, __dsub_rn(ai, a0)), twopi);
@@ -558,15 +560,26 @@ __device__ __forceinline__ uint32_t noiseless_digit(double2 v0, double2 vi, int 
     return (uint32_t)kq;
 }
 
+__device__ __forceinline__ uint32_t noiseless_digit(double2 v0, double2 vi, int q) {
+    const double m0 = fmax(fabs(v0.x), fabs(v0.y)), mi = fmax(fabs(vi.x), fabs(vi.y));
+    if (m0 > 1.0e-150 && m0 < 1.0e150 && mi > 1.0e-150 && mi < 1.0e150) {
+        const double wr = vi.x * v0.x + vi.y * v0.y, wi = vi.y * v0.x - vi.x * v0.y;
+        const int f = fast_sector(wr, wi, q);
+        if (f >= 0) return (uint32_t)f;
+    }
+    return noiseless_slow(v0, vi, q);
+}
+
 // Kernel A. Wave w of an instance owns bins [64 w, 64 w + 64): it classifies
 // the ones this pass scans (mark == pass), prefetching
 // the next bin's column while it works on the current one, and buffers its
 // singletons in LDS. Then one lane per buffered singleton registers it in the
 // hash table, so the atomics of up to 16 registrations are in flight
-// together, and the pass counters take one atomic per wave. Three waves per
-// SIMD fit without scratch (the FP64 atan2 is the largest register user).
+// together, and the pass counters take one atomic per wave. Four waves per
+// SIMD fit without scratch (at most 128 VGPRs): the FP64 atan2 rules, whose
+// registers would otherwise be reserved across the whole loop, are out of line.
 template <int NR>
-__global__ __launch_bounds__(kBinThreads) __attribute__((amdgpu_waves_per_eu(3))) void k_peel_detect(const qsft_peel_desc* __restrict__ descs,
+__global__ __launch_bounds__(kBinThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_peel_detect(const qsft_peel_desc* __restrict__ descs,
                                                              const int32_t* __restrict__ pre, int n_inst, int it) {
     __shared__ double2 tab[QSFT_MAX_Q];
     __shared__ __align__(16) uint32_t kws[kWaves][QSFT_MAX_N / 4];
