@@ -54,6 +54,9 @@ def parse():
     ap.add_argument("--streams", choices=("shared", "per-batch"), default="shared",
                     help="shared: both resident batches on one stream (the peeling tail on a second); "
                          "per-batch: each batch on its own stream, so their kernels interleave")
+    ap.add_argument("--fuse-from", default="3",
+                    help="first peeling pass of the fused tail kernel (one workgroup per instance looping over "
+                         "the remaining passes); 'none': one detect/apply launch pair per pass")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (a known-size reference for "
                          "hardware byte counters)")
@@ -131,7 +134,9 @@ def main():
     spec_sets = [[E.random_spec(q, n, S, b, C, R, snr, rng=rng) for _ in range(args.batch)] for _ in range(2)]
     # two resident batches alternate: while the host reads back one (its pack
     # kernels write the rows straight into pinned host memory), the GPU runs the other
-    engines = [E.Batch(sp, precision="fp32", own_stream=args.streams == "per-batch") for sp in spec_sets]
+    fuse = None if args.fuse_from == "none" else int(args.fuse_from)
+    engines = [E.Batch(sp, precision="fp32", own_stream=args.streams == "per-batch", fuse_from=fuse)
+               for sp in spec_sets]
     eng, specs = engines[0], spec_sets[0]
 
     log("warmup", args.quiet)

@@ -334,6 +334,12 @@ int qsft_peel_init(const qsft_peel_desc* dev_descs, const int32_t* dev_pre_bins,
                    int n_inst, int blocks_bins, int blocks_slots, void* stream);
 int qsft_peel_iteration(const qsft_peel_desc* dev_descs, const int32_t* dev_pre_bins, int n_inst,
                         int blocks_bins, int max_p, int iteration, void* stream);
+/* Passes it0 .. it_end - 1 of the same loop (qsft/qsft.py:142-255) in one
+ * launch, one workgroup per instance, for the tail of the loop where most
+ * instances have stopped and the rest touch few bins per pass: the same
+ * results as qsft_peel_iteration for each of those passes, without a launch
+ * per pass. A loop that stops leaves at once; 0 <= it0 <= it_end <= 15. */
+int qsft_peel_passes(const qsft_peel_desc* dev_descs, int n_inst, int max_p, int it0, int it_end, void* stream);
 /* word: QSFT_FIN_FLUSH first subtracts the balls the last pass queued, so U
  * ends as the reference's loop leaves it; without it U keeps them queued
  * (the recovered rows are the same either way). With QSFT_PEEL_SPARSE_U the

@@ -278,8 +278,8 @@ __device__ __forceinline__ void row_phases(const qsft_peel_desc& d, const QSFT_G
     }
 }
 
-// A queued ball: its entry, the next entry of its queue and its rho (scalar
-// loads: the previous pass's arrays are not written by this kernel). Its k row
+// A queued ball: its entry, the next entry of its queue and its rho (see
+// ld_prev for how the previous pass's arrays are read). Its k row
 // goes into an LDS buffer of the wave, either by an LDS-direct load issued a
 // bin ahead (prefetch_krow, waited for with the column) or by plain loads
 // (load_krow_now).
@@ -288,16 +288,33 @@ struct Pending {
     double rx, ry;
 };
 
+// A load of the previous pass's arrays. Written by an earlier launch: a scalar
+// load. Written earlier in the same launch (PERSIST, the fused tail kernel): a
+// vector load, since vector stores do not update the scalar cache.
+template <bool PERSIST>
+__device__ __forceinline__ int32_t ld_prev(const int32_t* p) {
+    if constexpr (PERSIST) return __hip_atomic_load(const_cast<int32_t*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    else return *cptr(p);
+}
+template <bool PERSIST>
+__device__ __forceinline__ double ld_prev(const double* p) {
+    if constexpr (PERSIST)
+        return __builtin_bit_cast(double, __hip_atomic_load(reinterpret_cast<uint64_t*>(const_cast<double*>(p)),
+                                                            __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP));
+    else return *cptr(p);
+}
+
+template <bool PERSIST>
 __device__ __forceinline__ Pending pending_head(const qsft_peel_desc& d, int ent, int from) {
     ent = __builtin_amdgcn_readfirstlane(ent);                     // wave-uniform
     Pending pd{ent, -1, 0.0, 0.0};
     if (ent < 0) return pd;
     const size_t CQ = (size_t)d.C * d.Q;
     const int sbin = ent / d.C;                                    // the ball's detection bin
-    const QSFT_CONST double* rprev = cptr(d.rho_bin) + (size_t)(from & 1) * CQ * 2;
-    pd.rx = rprev[2 * (size_t)sbin];
-    pd.ry = rprev[2 * (size_t)sbin + 1];
-    pd.next = cptr(d.pend_next)[ent];
+    const double* rprev = d.rho_bin + (size_t)(from & 1) * CQ * 2;
+    pd.rx = ld_prev<PERSIST>(rprev + 2 * (size_t)sbin);
+    pd.ry = ld_prev<PERSIST>(rprev + 2 * (size_t)sbin + 1);
+    pd.next = ld_prev<PERSIST>(d.pend_next + ent);
     return pd;
 }
 
@@ -364,7 +381,7 @@ __device__ __forceinline__ void subtract_ball(const qsft_peel_desc& d, const QSF
     wave_lds_sync();                                               // the k row buffer is rewritten next
 }
 
-template <int NR>
+template <bool PERSIST, int NR>
 __device__ __forceinline__ void apply_pending(const qsft_peel_desc& d, int t, int c, int from, Pending pd,
                                               double2 (&u)[NR], const uint32_t* kfirst, uint32_t* kk,
                                               const double2* tab, const ModQ& mq, int lane, const int8_t* offs,
@@ -379,7 +396,7 @@ __device__ __forceinline__ void apply_pending(const qsft_peel_desc& d, int t, in
     while (rest >= 0 && m < kSortedPending) {
         if (lane == m) mine = rest;
         ++m;
-        rest = __builtin_amdgcn_readfirstlane(cptr(d.pend_next)[rest]);
+        rest = __builtin_amdgcn_readfirstlane(ld_prev<PERSIST>(d.pend_next + rest));
     }
     int rank = 0;                                                  // position in ascending order
     for (int a = 0; a < m; ++a) {
@@ -399,7 +416,7 @@ __device__ __forceinline__ void apply_pending(const qsft_peel_desc& d, int t, in
         if (e == pd.ent) {
             subtract_ball(d, Dc, pd, u, kfirst, tab, mq, lane, offs, offs_staged);
         } else {
-            const Pending pa = pending_head(d, e, from);
+            const Pending pa = pending_head<PERSIST>(d, e, from);
             if (a >= m) rest = pa.next;
             load_krow_now(d, e, from, lane, kk);
             subtract_ball(d, Dc, pa, u, kk, tab, mq, lane, offs, offs_staged);
@@ -480,9 +497,9 @@ __global__ __launch_bounds__(kSlotThreads) void k_peel_flush(const qsft_peel_des
         double2 u[kRowsPerLane];
         if (sparse_u(d) && !d.written[t]) rebuild_col(d, t / d.Q, t % d.Q, u, lane);
         else load_col(u, reinterpret_cast<const double2*>(d.U) + (size_t)t * P, P, lane);
-        const Pending pd = pending_head(d, d.pend_head[t], (int)iters - 1);
+        const Pending pd = pending_head<false>(d, d.pend_head[t], (int)iters - 1);
         load_krow_now(d, pd.ent, (int)iters - 1, lane, kfirst[wave]);
-        apply_pending(d, t, t / d.Q, (int)iters - 1, pd, u, kfirst[wave], kws[wave], tab, mq, lane, offbuf, staged);
+        apply_pending<false>(d, t, t / d.Q, (int)iters - 1, pd, u, kfirst[wave], kws[wave], tab, mq, lane, offbuf, staged);
         store_col(d, t, u, lane);
     }
 }
@@ -570,38 +587,51 @@ __device__ __forceinline__ uint32_t noiseless_digit(double2 v0, double2 vi, int 
     return noiseless_slow(v0, vi, q);
 }
 
-// Kernel A. Wave w of an instance owns bins [64 w, 64 w + 64): it classifies
-// the ones this pass scans (mark == pass), prefetching
-// the next bin's column while it works on the current one, and buffers its
-// singletons in LDS. Then one lane per buffered singleton registers it in the
-// hash table, so the atomics of up to 16 registrations are in flight
-// together, and the pass counters take one atomic per wave. Four waves per
-// SIMD fit without scratch (at most 128 VGPRs): the FP64 atan2 rules, whose
-// registers would otherwise be reserved across the whole loop, are out of line.
+// Per-wave LDS of the detection: the k digits, two column slots with the
+// first ball queued on each, and the singletons buffered for registration.
 template <int NR>
-__global__ __launch_bounds__(kBinThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_peel_detect(const qsft_peel_desc* __restrict__ descs,
-                                                             const int32_t* __restrict__ pre, int n_inst, int it) {
-    __shared__ double2 tab[QSFT_MAX_Q];
-    __shared__ __align__(16) uint32_t kws[kWaves][QSFT_MAX_N / 4];
-    __shared__ __align__(16) PendingSlot pslot[kWaves][2];                // first queued ball, per column slot
-    __shared__ __align__(16) double2 colw[kWaves][2][64 * NR];   // current / next column
-    __shared__ uint64_t sb_h[kWaves][kBinsPerWave];
-    __shared__ double2 sb_rho[kWaves][kBinsPerWave];
-    __shared__ int32_t sb_t[kWaves][kBinsPerWave];
-    __shared__ __align__(16) int8_t offbuf[kStageOffBytes];
-    __shared__ __align__(16) int8_t mtbuf[kStageMtBytes];
-    const int inst = find_inst(pre, n_inst, blockIdx.x);
-    const qsft_peel_desc& d = descs[inst];
+struct DetectWaveLds {
+    __align__(16) uint32_t kws[QSFT_MAX_N / 4];
+    __align__(16) PendingSlot pslot[2];                            // first queued ball, per column slot
+    __align__(16) double2 colw[2][64 * NR];                        // current / next column
+    uint64_t sb_h[kBinsPerWave];
+    double2 sb_rho[kBinsPerWave];
+    int32_t sb_t[kBinsPerWave];
+};
+
+// The wave's bins due this pass among the 64 from `base` (bit i: bin base + i).
+// A held multiton (a multiton last pass that no ball has been queued on since)
+// keeps its column, so it is a multiton again: it is counted (nheld) and
+// re-stamped without being read.
+__device__ __forceinline__ uint64_t scan_marks(const qsft_peel_desc& d, int it, int base, uint32_t& nheld) {
+    const int lane = threadIdx.x & 63;
+    QSFT_GLOBAL int32_t* __restrict__ mark = gptr(d.mark);
+    const int mk = (base + lane < d.C * d.Q) ? mark[base + lane] : -1;
+    const bool due = (mk & ~kHeldMulti) == it;
+    const bool held = due && (mk & kHeldMulti) != 0;
+    nheld = (uint32_t)__popcll(__ballot(held));
+    if (held) mark[base + lane] = (it + 1) | kHeldMulti;
+    return __ballot(due && !held);
+}
+
+// One wave classifies the due bins `todo` among the 64 from `base` and
+// registers its singletons (kernel A's body; the fused tail kernel runs it
+// with PERSIST). tab, offbuf and mtbuf are the block's staged tables.
+template <int NR, bool PERSIST>
+__device__ __forceinline__ void detect_bins(const qsft_peel_desc& d, int it, int base, uint64_t todo, uint32_t nheld,
+                                            DetectWaveLds<NR>& W, const double2* tab, const int8_t* offbuf,
+                                            bool offs_staged, const int8_t* mtbuf, bool mt_staged) {
     uint32_t* __restrict__ ctr = d.ctr;
-    if (it >= d.max_iter) return;                                  // beyond the bound workspace
-    if (cptr(ctr)[it * QSFT_CTR_ITER_STRIDE + 3] == 0) return;    // loop already stopped (set by the
-                                                                   // previous pass; scalar load)
+    const int lane = threadIdx.x & 63;
+    if (todo == 0) {
+        if (lane == 0 && nheld) atomicAdd(ctr + it * QSFT_CTR_ITER_STRIDE + 1, nheld);
+        return;
+    }
     const int q = d.q, n = d.n, b = d.b, C = d.C, P = d.P, Q = d.Q, n_pad = d.n_pad;
     const int nw = n_pad >> 2;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int CQ = C * Q;
     QSFT_GLOBAL int32_t* __restrict__ mark = gptr(d.mark);
-    uint32_t* kk = kws[wave];
+    uint32_t* kk = W.kws;
     int8_t* kb8 = reinterpret_cast<int8_t*>(kk);
     const ModQ mq((uint32_t)q);
     const double thr = d.cutoff * (double)P;
@@ -614,28 +644,6 @@ __global__ __launch_bounds__(kBinThreads) __attribute__((amdgpu_waves_per_eu(4))
         wdig = 1;
         for (int e = lane + 1; e < b; ++e) wdig *= q;
     }
-    const int base = ((blockIdx.x - pre[inst]) * kWaves + wave) * kBinsPerWave;
-    // The wave's bins due this pass. A held multiton (a multiton last pass that
-    // no ball has been queued on since) keeps its column, so it is a multiton
-    // again: it is counted and re-stamped without being read.
-    const int mk = (base + lane < CQ) ? mark[base + lane] : -1;
-    const bool due = (mk & ~kHeldMulti) == it;
-    const bool held = due && (mk & kHeldMulti) != 0;
-    uint64_t todo = __ballot(due && !held);
-    const uint32_t nheld = (uint32_t)__popcll(__ballot(held));
-    if (held) mark[base + lane] = (it + 1) | kHeldMulti;
-    if (!__syncthreads_or(todo != 0)) {                            // nothing to read in this block
-        if (lane == 0 && nheld) atomicAdd(ctr + it * QSFT_CTR_ITER_STRIDE + 1, nheld);
-        return;
-    }
-    fill_wtable_d(tab, q);
-    const bool offs_staged = stage_offsets(d, offbuf);
-    const bool mt_staged = stage_mt(d, mtbuf);
-    __syncthreads();
-    if (todo == 0) {
-        if (lane == 0 && nheld) atomicAdd(ctr + it * QSFT_CTR_ITER_STRIDE + 1, nheld);
-        return;
-    }
     const uint32_t scanned = (uint32_t)__popcll(todo);
     int nz = (int)nheld, ns = 0;                                   // wave-uniform
     // sparse U: due bins whose column the assembler skipped (only this wave
@@ -645,7 +653,7 @@ __global__ __launch_bounds__(kBinThreads) __attribute__((amdgpu_waves_per_eu(4))
     int c = base / Q;                                              // group of the current bin
 
     // queue heads of the wave's bins (balls the previous pass peeled into them)
-    const int qhead = (it > 0 && due && !held) ? gptr(d.pend_head)[base + lane] : -1;
+    const int qhead = (it > 0 && ((todo >> lane) & 1)) ? gptr(d.pend_head)[base + lane] : -1;
     // Bins stream through two LDS slots per wave: each holds a column and the
     // first queued ball (k row, rho, next entry), loaded by LDS-direct loads
     // (no registers held) a bin ahead. A step issues the next bin's loads,
@@ -656,11 +664,11 @@ __global__ __launch_bounds__(kBinThreads) __attribute__((amdgpu_waves_per_eu(4))
     const int nrow = (P + 63) >> 6;                                // loads per column
     const uint32_t rowoff = (uint32_t)min(lane, P - 1) * 16u;     // rows past P re-read row P - 1
     auto issue = [&](int tb, int ent, int slot) {
-        if (ent >= 0) prefetch_pending(d, ent, it - 1, lane, &pslot[wave][slot]);
+        if (ent >= 0) prefetch_pending(d, ent, it - 1, lane, &W.pslot[slot]);
         const double2* cb = Ub + (size_t)tb * P;
         for (int r = 0; r < nrow; ++r)
             dma16(cb, min(64 * r * 16u + rowoff, (uint32_t)(P - 1) * 16u),
-                  __builtin_amdgcn_readfirstlane(lds_addr(&colw[wave][slot][64 * r])));
+                  __builtin_amdgcn_readfirstlane(lds_addr(&W.colw[slot][64 * r])));
     };
     int cur = __builtin_ctzll(todo);
     int cslot = 0;
@@ -670,13 +678,13 @@ __global__ __launch_bounds__(kBinThreads) __attribute__((amdgpu_waves_per_eu(4))
     __asm__ volatile("" ::: "memory");
     for (;;) {
         const int t = base + cur;
-        double2* cw = colw[wave][cslot];
+        double2* cw = W.colw[cslot];
         Pending pd{curent, -1, 0.0, 0.0};
         if (curent >= 0) {
-            const double2 rv = pslot[wave][cslot].rho;
+            const double2 rv = W.pslot[cslot].rho;
             pd.rx = rv.x;
             pd.ry = rv.y;
-            pd.next = pslot[wave][cslot].next;
+            pd.next = W.pslot[cslot].next;
         }
         todo &= todo - 1;
         const bool more = todo != 0;
@@ -698,7 +706,7 @@ __global__ __launch_bounds__(kBinThreads) __attribute__((amdgpu_waves_per_eu(4))
             if (rebuilt) rebuild_col(d, c, j, u, lane);            // (its prefetched copy is not U)
         }
         // balls queued on this bin by the previous pass
-        apply_pending(d, t, c, it - 1, pd, u, pslot[wave][cslot].krow, kk, tab, mq, lane, offbuf, offs_staged);
+        apply_pending<PERSIST>(d, t, c, it - 1, pd, u, W.pslot[cslot].krow, kk, tab, mq, lane, offbuf, offs_staged);
         int mval = 0;                                              // mark to store (0: none)
         bool single = false;
         double rho_x = 0.0, rho_y = 0.0;
@@ -782,9 +790,9 @@ __global__ __launch_bounds__(kBinThreads) __attribute__((amdgpu_waves_per_eu(4))
             if (single) {
                 const uint64_t hk = hash_k(kk, nw, lane);
                 if (lane == 0) {
-                    sb_t[wave][ns] = t;
-                    sb_h[wave][ns] = hk;
-                    sb_rho[wave][ns] = make_double2(rho_x, rho_y);
+                    W.sb_t[ns] = t;
+                    W.sb_h[ns] = hk;
+                    W.sb_rho[ns] = make_double2(rho_x, rho_y);
                 }
                 ++ns;
             }
@@ -824,9 +832,9 @@ __global__ __launch_bounds__(kBinThreads) __attribute__((amdgpu_waves_per_eu(4))
     uint64_t tm = 0;
     double2 rho = make_double2(0.0, 0.0);
     if (lane < ns) {
-        t = sb_t[wave][lane];
-        rho = sb_rho[wave][lane];
-        const uint64_t hv = sb_h[wave][lane];
+        t = W.sb_t[lane];
+        rho = W.sb_rho[lane];
+        const uint64_t hv = W.sb_h[lane];
         const uint64_t tg = hv | 1ull;
         const uint32_t mask = (1u << d.hcap_log2) - 1u;
         uint32_t pos = (uint32_t)(hv >> 32) & mask;
@@ -876,11 +884,50 @@ __global__ __launch_bounds__(kBinThreads) __attribute__((amdgpu_waves_per_eu(4))
     }
 }
 
+
+// Kernel A. Wave w of an instance owns bins [64 w, 64 w + 64): it classifies
+// the ones this pass scans (mark == pass), prefetching
+// the next bin's column while it works on the current one, and buffers its
+// singletons in LDS. Then one lane per buffered singleton registers it in the
+// hash table, so the atomics of up to 16 registrations are in flight
+// together, and the pass counters take one atomic per wave. Four waves per
+// SIMD fit without scratch (at most 128 VGPRs): the FP64 atan2 rules, whose
+// registers would otherwise be reserved across the whole loop, are out of line.
+template <int NR>
+__global__ __launch_bounds__(kBinThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_peel_detect(const qsft_peel_desc* __restrict__ descs,
+                                                             const int32_t* __restrict__ pre, int n_inst, int it) {
+    __shared__ double2 tab[QSFT_MAX_Q];
+    __shared__ DetectWaveLds<NR> wl[kWaves];
+    __shared__ __align__(16) int8_t offbuf[kStageOffBytes];
+    __shared__ __align__(16) int8_t mtbuf[kStageMtBytes];
+    const int inst = find_inst(pre, n_inst, blockIdx.x);
+    const qsft_peel_desc& d = descs[inst];
+    uint32_t* __restrict__ ctr = d.ctr;
+    if (it >= d.max_iter) return;                                  // beyond the bound workspace
+    if (cptr(ctr)[it * QSFT_CTR_ITER_STRIDE + 3] == 0) return;    // loop already stopped (set by the
+                                                                   // previous pass; scalar load)
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int base = ((blockIdx.x - pre[inst]) * kWaves + wave) * kBinsPerWave;
+    uint32_t nheld;
+    const uint64_t todo = scan_marks(d, it, base, nheld);
+    if (!__syncthreads_or(todo != 0)) {                            // nothing to read in this block
+        if (lane == 0 && nheld) atomicAdd(ctr + it * QSFT_CTR_ITER_STRIDE + 1, nheld);
+        return;
+    }
+    fill_wtable_d(tab, d.q);
+    const bool offs_staged = stage_offsets(d, offbuf);
+    const bool mt_staged = stage_mt(d, mtbuf);
+    __syncthreads();
+    detect_bins<NR, false>(d, it, base, todo, nheld, wl[wave], tab, offbuf, offs_staged, mtbuf, mt_staged);
+}
+
 constexpr int kKeyChunks = QSFT_MAX_N / 16;                        // 16-byte chunks of a k row
 
 // The checks and queue pushes of kernel B for the singleton found in bin t.
+// mt: the block's LDS copy of every group's M^T rows (stage_mt), or null to
+// read them from global memory.
 __device__ __forceinline__ void apply_singleton(const qsft_peel_desc& d, int t, int it, int q, int b, int C, int Q,
-                                                int n_pad, int nq, uint32_t* ctr) {
+                                                int n_pad, int nq, uint32_t* ctr, const int8_t* mt) {
     const int CQ = C * Q;
     const ModQ mq((uint32_t)q);
     const int8_t* __restrict__ kcur = d.kbin + (size_t)(it & 1) * CQ * n_pad;
@@ -920,16 +967,31 @@ __device__ __forceinline__ void apply_singleton(const qsft_peel_desc& d, int t, 
     for (int l = 0; l < C; ++l) {
         int jl = 0;
         for (int dg = 0; dg < b; ++dg) {
-            const uint4* __restrict__ mrow = reinterpret_cast<const uint4*>(d.Mt + ((size_t)l * b + dg) * n_pad);
+            const size_t mo = ((size_t)l * b + dg) * n_pad;
             int acc = 0;
+            if (mt) {                                              // (block-uniform)
+                const uint4* mrow = reinterpret_cast<const uint4*>(mt + mo);
 #pragma unroll
-            for (int w = 0; w < kKeyChunks; ++w) {
-                if (w < nq) {
-                    const uint4 mv = mrow[w];
-                    acc = __builtin_amdgcn_sdot4((int)mv.x, (int)k[w].x, acc, false);
-                    acc = __builtin_amdgcn_sdot4((int)mv.y, (int)k[w].y, acc, false);
-                    acc = __builtin_amdgcn_sdot4((int)mv.z, (int)k[w].z, acc, false);
-                    acc = __builtin_amdgcn_sdot4((int)mv.w, (int)k[w].w, acc, false);
+                for (int w = 0; w < kKeyChunks; ++w) {
+                    if (w < nq) {
+                        const uint4 mv = mrow[w];
+                        acc = __builtin_amdgcn_sdot4((int)mv.x, (int)k[w].x, acc, false);
+                        acc = __builtin_amdgcn_sdot4((int)mv.y, (int)k[w].y, acc, false);
+                        acc = __builtin_amdgcn_sdot4((int)mv.z, (int)k[w].z, acc, false);
+                        acc = __builtin_amdgcn_sdot4((int)mv.w, (int)k[w].w, acc, false);
+                    }
+                }
+            } else {
+                const uint4* __restrict__ mrow = reinterpret_cast<const uint4*>(d.Mt + mo);
+#pragma unroll
+                for (int w = 0; w < kKeyChunks; ++w) {
+                    if (w < nq) {
+                        const uint4 mv = mrow[w];
+                        acc = __builtin_amdgcn_sdot4((int)mv.x, (int)k[w].x, acc, false);
+                        acc = __builtin_amdgcn_sdot4((int)mv.y, (int)k[w].y, acc, false);
+                        acc = __builtin_amdgcn_sdot4((int)mv.z, (int)k[w].z, acc, false);
+                        acc = __builtin_amdgcn_sdot4((int)mv.w, (int)k[w].w, acc, false);
+                    }
                 }
             }
             jl = jl * q + (int)mq((uint32_t)acc);
@@ -949,35 +1011,133 @@ __device__ __forceinline__ void apply_singleton(const qsft_peel_desc& d, int t, 
 // M^T rows; one atomicExch per target) and stamps the target for the next
 // pass, whose kernel A subtracts it from the column it owns.
 
+// Loop control after pass `it` (one thread; its detection counters are final):
+// the iteration and peel counts, and whether pass it + 1 runs: only if this
+// pass ran and found a singleton and a multiton, below the peel cap.
+template <bool PERSIST>
+__device__ __forceinline__ void loop_control(const qsft_peel_desc& d, int it, uint32_t active) {
+    uint32_t* __restrict__ ctr = d.ctr;
+    auto rd = [&](int w) -> uint32_t {
+        if constexpr (PERSIST) return __hip_atomic_load(ctr + w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        else return ctr[w];
+    };
+    const uint32_t ns = rd(it * QSFT_CTR_ITER_STRIDE + 0);
+    const uint32_t nm = rd(it * QSFT_CTR_ITER_STRIDE + 1);
+    const uint32_t nb = rd(it * QSFT_CTR_ITER_STRIDE + 2);
+    uint32_t peeled = rd(QSFT_CTR_PEELED);
+    if (active) {
+        ctr[QSFT_CTR_ITERS] = (uint32_t)(it + 1);
+        peeled = (peeled + nb < peeled) ? 0xFFFFFFFFu : peeled + nb;
+        ctr[QSFT_CTR_PEELED] = peeled;
+    }
+    const uint32_t next = (active && ns > 0 && nm > 0 && (double)peeled < d.peel_cap) ? 1u : 0u;
+    if (it + 1 < 16) ctr[(it + 1) * QSFT_CTR_ITER_STRIDE + 3] = next;
+}
+
 __global__ __launch_bounds__(kBinThreads) void k_peel_apply(const qsft_peel_desc* __restrict__ descs,
                                                             const int32_t* __restrict__ pre, int n_inst, int it) {
+    __shared__ __align__(16) int8_t mtbuf[kStageMtBytes];
     const int inst = find_inst(pre, n_inst, blockIdx.x);
     const qsft_peel_desc& d = descs[inst];
     uint32_t* __restrict__ ctr = d.ctr;
     const int local = blockIdx.x - pre[inst];
     if (it >= d.max_iter) return;
     const uint32_t active = cptr(ctr)[it * QSFT_CTR_ITER_STRIDE + 3];   // written before this kernel
-    if (local == 0 && threadIdx.x == 0) {
-        // loop control for the next pass (the counters of this pass are final)
-        const uint32_t ns = ctr[it * QSFT_CTR_ITER_STRIDE + 0];
-        const uint32_t nm = ctr[it * QSFT_CTR_ITER_STRIDE + 1];
-        const uint32_t nb = ctr[it * QSFT_CTR_ITER_STRIDE + 2];
-        uint32_t peeled = ctr[QSFT_CTR_PEELED];
-        if (active) {
-            ctr[QSFT_CTR_ITERS] = (uint32_t)(it + 1);
-            peeled = (peeled + nb < peeled) ? 0xFFFFFFFFu : peeled + nb;
-            ctr[QSFT_CTR_PEELED] = peeled;
-        }
-        const uint32_t next = (active && ns > 0 && nm > 0 && (double)peeled < d.peel_cap) ? 1u : 0u;
-        if (it + 1 < 16) ctr[(it + 1) * QSFT_CTR_ITER_STRIDE + 3] = next;
-    }
+    if (local == 0 && threadIdx.x == 0) loop_control<false>(d, it, active);   // the pass's counters are final
     if (!active) return;
     const int q = d.q, b = d.b, C = d.C, Q = d.Q, n_pad = d.n_pad;
     const int nq = n_pad >> 4;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int CQ = C * Q;
     const int t = (local * kWaves + wave) * kBinsPerWave + lane;
-    if (t < CQ && d.sstamp[t] == it + 1) apply_singleton(d, t, it, q, b, C, Q, n_pad, nq, ctr);   // no cross-lane work
+    const bool mine = t < CQ && d.sstamp[t] == it + 1;
+    if (!__syncthreads_or(mine)) return;                           // (active is the same for the whole block)
+    const bool mt_staged = stage_mt(d, mtbuf);                     // the targets' bin digits read M^T from LDS
+    __syncthreads();
+    if (mine) apply_singleton(d, t, it, q, b, C, Q, n_pad, nq, ctr, mt_staged ? mtbuf : nullptr);   // no cross-lane work
+}
+
+// Passes it0 .. it_end - 1 in one launch, one workgroup per instance (the
+// fused tail of the loop: after the first passes most loops have stopped and
+// the rest touch a few bins per pass, so per-pass launches would cost more
+// than the work). Each pass runs kernel A's body over the instance's 64-bin
+// groups (wave w takes groups w, w + kW, ...; the marks of kTailBatch groups
+// are loaded together), a barrier, the loop control, kernel B's body over the
+// instance's bins (kTailBatch stamps loaded together), and a barrier. The stop
+// flag is read by one thread and broadcast through LDS, so every thread
+// leaves the loop at the same pass. Only this workgroup touches its
+// instance's arrays, so the barriers' workgroup-scope ordering is all the
+// passes need; arrays written earlier in the launch are read by vector loads
+// (PERSIST), never through the scalar cache.
+template <int NR>
+constexpr int tail_waves() { return NR == 2 ? 12 : 8; }           // no scratch: 170 / 256 VGPRs per lane
+constexpr int kTailBatch = 8;
+
+template <int NR>
+__global__ __launch_bounds__(64 * tail_waves<NR>()) void k_peel_tail(const qsft_peel_desc* __restrict__ descs,
+                                                                  int it0, int it_end) {
+    constexpr int kW = tail_waves<NR>();
+    __shared__ double2 tab[QSFT_MAX_Q];
+    __shared__ DetectWaveLds<NR> wl[kW];
+    __shared__ __align__(16) int8_t offbuf[kStageOffBytes];
+    __shared__ __align__(16) int8_t mtbuf[kStageMtBytes];
+    __shared__ uint64_t tdue[kW][kTailBatch];                      // due masks of the wave's scanned groups
+    __shared__ uint32_t theld[kW][kTailBatch];
+    __shared__ uint32_t go;
+    const qsft_peel_desc& d = descs[blockIdx.x];
+    uint32_t* __restrict__ ctr = d.ctr;
+    const int end = min(it_end, min(d.max_iter, 15));
+    if (it0 >= end) return;                                        // (block-uniform)
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int CQ = d.C * d.Q, n_pad = d.n_pad;
+    constexpr int kGroupStride = kW * kBinsPerWave;
+    fill_wtable_d(tab, d.q);
+    const bool offs_staged = stage_offsets(d, offbuf);
+    const bool mt_staged = stage_mt(d, mtbuf);
+    for (int it = it0; it < end; ++it) {
+        if (threadIdx.x == 0)
+            go = __hip_atomic_load(ctr + it * QSFT_CTR_ITER_STRIDE + 3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __syncthreads();                                           // go and (first pass) the tables
+        if (go == 0) break;                                        // the loop has stopped (every thread)
+        for (int g0 = wave * kBinsPerWave; g0 < CQ; g0 += kGroupStride * kTailBatch) {
+            uint64_t due[kTailBatch];
+            uint32_t nh[kTailBatch];
+#pragma unroll
+            for (int i = 0; i < kTailBatch; ++i) due[i] = scan_marks(d, it, g0 + i * kGroupStride, nh[i]);
+            if (lane == 0) {
+#pragma unroll
+                for (int i = 0; i < kTailBatch; ++i) {
+                    tdue[wave][i] = due[i];
+                    theld[wave][i] = nh[i];
+                }
+            }
+            wave_lds_sync();
+            for (int i = 0; i < kTailBatch; ++i) {
+                const uint64_t todo = tdue[wave][i];
+                const uint32_t nheld = theld[wave][i];
+                if (todo != 0 || nheld != 0)
+                    detect_bins<NR, true>(d, it, g0 + i * kGroupStride, todo, nheld, wl[wave], tab, offbuf,
+                                          offs_staged, mtbuf, mt_staged);
+            }
+            wave_lds_sync();                                       // tdue is rewritten next
+        }
+        __syncthreads();                                           // every detection registered, counters final
+        if (threadIdx.x == 0) loop_control<true>(d, it, 1u);
+        for (int t0 = threadIdx.x; t0 < CQ; t0 += blockDim.x * kTailBatch) {
+            uint32_t hit = 0;
+#pragma unroll
+            for (int i = 0; i < kTailBatch; ++i) {
+                const int t = t0 + i * (int)blockDim.x;
+                if (t < CQ && gptr(d.sstamp)[t] == it + 1) hit |= 1u << i;
+            }
+            while (hit) {
+                const int t = t0 + __builtin_ctz(hit) * (int)blockDim.x;
+                hit &= hit - 1;
+                apply_singleton(d, t, it, d.q, d.b, d.C, d.Q, n_pad, n_pad >> 4, ctr, mt_staged ? mtbuf : nullptr);
+            }
+        }
+        __syncthreads();                                           // queues and stamps for the next pass
+    }
 }
 
 // Output of one key: the running mean of its records in time order. Up to
@@ -1323,6 +1483,19 @@ extern "C" int qsft_peel_finalize(const qsft_peel_desc* d, const int32_t* pre_sl
     return qsft_hip_status(hipGetLastError(), "qsft_peel_finalize");
 }
 
+extern "C" int qsft_peel_passes(const qsft_peel_desc* d, int n_inst, int max_p, int it0, int it_end, void* stream) {
+    if (!d || n_inst < 1 || it0 < 0 || it_end > 15 || it0 > it_end || max_p < 1 || max_p > 64 * kRowsPerLane) {
+        qsft_set_error("qsft_peel_passes: bad arguments");
+        return QSFT_ERR_ARG;
+    }
+    if (it0 == it_end) return QSFT_OK;
+    hipStream_t st = (hipStream_t)stream;
+    if (max_p <= 128)
+        hipLaunchKernelGGL(k_peel_tail<2>, dim3(n_inst), dim3(64 * tail_waves<2>()), 0, st, d, it0, it_end);
+    else
+        hipLaunchKernelGGL(k_peel_tail<4>, dim3(n_inst), dim3(64 * tail_waves<4>()), 0, st, d, it0, it_end);
+    return qsft_hip_status(hipGetLastError(), "qsft_peel_passes");
+}
 extern "C" int qsft_peel_run(const qsft_peel_desc* d, const int32_t* pre_bins, const int32_t* pre_slots, int n_inst,
                              int blocks_bins, int blocks_slots, int max_p, int max_iter, void* stream) {
     if (max_iter < 1 || max_iter > 15) { qsft_set_error("qsft_peel_run: max_iter must be in [1, 15]"); return QSFT_ERR_ARG; }

@@ -109,7 +109,7 @@ class Batch:
     K1 shape)."""
 
     def __init__(self, specs, precision="fp32", max_iter=15, device=None, eval_mode=None, fused_screen=True,
-                 host_rows=True, tail_stream=True, own_stream=False, sparse_u=True):
+                 host_rows=True, tail_stream=True, own_stream=False, sparse_u=True, fuse_from=3):
         self.specs = list(specs)
         self.precision = precision
         if eval_mode is None:
@@ -176,6 +176,11 @@ class Batch:
         # launches; a launch that falls short (an instance whose loop runs to
         # the pass cap) is finished in fetch(), as the packed header says.
         self.max_iter = max_iter
+        # Passes from fuse_from on run as one kernel (one workgroup per
+        # instance, looping over the passes): a loop that stops costs nothing
+        # after that, and one that runs to the cap pays no launch per pass.
+        # None: one launch pair per pass, the count adapted as below.
+        self.fuse_from = fuse_from
         self.adaptive = self.tail is not None or self.stream is not None
         self.passes = max_iter
         self.launched = max_iter
@@ -213,8 +218,12 @@ class Batch:
         mark("dft")
         self.asm.launch(seed_offset)
         mark("assemble")
-        self.launched = self.passes if (self.adaptive and not host_loop) else self.max_iter
-        self.peel.run(host_loop=host_loop, tail_stream=self.tail, tail_event=self.tail_event, passes=self.launched)
+        if self.fuse_from is not None and not host_loop:
+            self.launched = self.max_iter
+            self.peel.run(tail_stream=self.tail, tail_event=self.tail_event, fuse_from=self.fuse_from)
+        else:
+            self.launched = self.passes if (self.adaptive and not host_loop) else self.max_iter
+            self.peel.run(host_loop=host_loop, tail_stream=self.tail, tail_event=self.tail_event, passes=self.launched)
         # one completion event per batch, re-recorded every launch (a copy that
         # waits on it captured the previous record when it was enqueued)
         if self.done is None:

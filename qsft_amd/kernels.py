@@ -480,17 +480,27 @@ class PeelBatch:
         """(written, noise_key) device addresses of instance i, for its assembler."""
         return self.descs[i].written, self.descs[i].noise_key
 
-    def run(self, host_loop=False, tail_stream=None, tail_event=None, passes=None):
+    def run(self, host_loop=False, tail_stream=None, tail_event=None, passes=None, fuse_from=None):
         """The peeling passes on the current stream, then the tail (last
         pass's queued balls, output table, pack). With `tail_stream` the tail
         runs there, after `tail_event` (recorded here on the current stream),
         so the current stream can go on with other work meanwhile. `passes`
         (< max_iter) launches only that many passes; the packed header then
-        says whether a loop wanted more (resume() runs the rest)."""
+        says whether a loop wanted more (resume() runs the rest). `fuse_from`
+        (without host_loop) launches passes fuse_from .. max_iter - 1 as one
+        kernel (qsft_peel_passes) after the per-pass launches before it."""
         st = stream_handle()
         n = len(self.descs)
         d, pb, ps = _vp(self.dptr), _vp(self.pre_b), _vp(self.pre_s)
         passes = self.max_iter if passes is None else min(int(passes), self.max_iter)
+        if fuse_from is not None and not host_loop:
+            fuse_from = max(0, min(int(fuse_from), self.max_iter))
+            L.check(lib.qsft_peel_init(d, pb, ps, n, self.tot_b, self.tot_s, st), "qsft_peel_init")
+            for it in range(fuse_from):
+                L.check(lib.qsft_peel_iteration(d, pb, n, self.tot_b, self.max_p, it, st), "qsft_peel_iteration")
+            L.check(lib.qsft_peel_passes(d, n, self.max_p, fuse_from, self.max_iter, st), "qsft_peel_passes")
+            self._tail(tail_stream, tail_event)
+            return
         if (not host_loop and tail_stream is None and self.fin_word == L.QSFT_FIN_FLUSH
                 and passes == self.max_iter):
             L.check(lib.qsft_peel_run(d, pb, ps, n, self.tot_b, self.tot_s, self.max_p, self.max_iter, st),

@@ -125,8 +125,8 @@ def test_short_pass_launch_is_finished_on_fetch():
     the rows equal those of a launch with every pass."""
     rng = np.random.default_rng(21)
     specs = [E.random_spec(4, 30, 200, 4, 3, 2, 15.0, rng=rng), E.random_spec(4, 20, 120, 4, 3, 2, None, rng=rng)]
-    full = E.Batch(specs, precision="fp32")
-    short = E.Batch(specs, precision="fp32")
+    full = E.Batch(specs, precision="fp32", fuse_from=None)
+    short = E.Batch(specs, precision="fp32", fuse_from=None)
     full.launch(seed_offset=3)
     ref = full.results()
     short.passes = 1
@@ -137,6 +137,45 @@ def test_short_pass_launch_is_finished_on_fetch():
         assert np.array_equal(ka, kb) and np.array_equal(oa, ob)
         assert np.array_equal(va, vb)
     assert short.passes <= short.max_iter and short.peel.last_iters >= 2
+
+
+@pytest.mark.parametrize("case", ["k1_mixed_snr", "p_over_128", "k1_to_cap"])
+def test_fused_tail_passes_match_per_pass_launches(case):
+    """Passes fuse_from .. 14 as one kernel (one workgroup per instance looping
+    over the passes, qsft_peel_passes) give what one launch pair per pass
+    gives: the same keys in the same order, the same bits, the same pass
+    counts, for loops that stop early and loops that run to the pass cap, with
+    a sparse (K1) or dense (P > 128) working transform."""
+    rng = np.random.default_rng(31)
+    if case == "k1_mixed_snr":
+        specs = [E.random_spec(4, 50, 1000, 6, 3, 2, snr, rng=rng) for snr in (0.0, 3.0, 5.0, 10.0, 10.0, 20.0, None)]
+    elif case == "p_over_128":
+        specs = [E.random_spec(4, 100, 400, 5, 3, 2, snr, rng=rng) for snr in (3.0, 10.0, None)]
+    else:
+        # the two batches of 64 K1 transforms at 10 dB that bench.py serves, at
+        # its noise draws: one of them leaves a loop that still finds
+        # singletons and multitons at every pass up to the cap
+        rng = np.random.default_rng(1234)
+        sets = [[E.random_spec(4, 50, 1000, 6, 3, 2, 10.0, rng=rng) for _ in range(64)] for _ in range(2)]
+    tries = [(specs, 5)] if case != "k1_to_cap" else [(sets[s % 2], 1000 + s) for s in range(8)]
+    for specs, off in tries:
+        ref = E.Batch(specs, precision="fp32", fuse_from=None)
+        ref.launch(seed_offset=off)
+        want = ref.results()
+        iters = ref.peel.last_iters
+        if case != "k1_to_cap" or iters == ref.max_iter:
+            break
+    print(case, "seed offset", off, "passes", iters, flush=True)
+    if case == "k1_to_cap":
+        assert iters == ref.max_iter
+    for fuse_from in (1, 2, 4):
+        bt = E.Batch(specs, precision="fp32", fuse_from=fuse_from)
+        bt.launch(seed_offset=off)
+        got = bt.results()
+        assert bt.peel.last_iters == iters and bt.peel.last_unfinished == 0
+        for a, b in zip(got, want):
+            _identical(a, b)
+            assert np.array_equal(a[2], b[2])                         # detection order
 
 
 def _written_flags(bt, i):
