@@ -200,21 +200,20 @@ __device__ __forceinline__ int row_sum_i(int v) {
 // Rows per lane of a bin column held by one wave: row p = lane + 64 r (P <= 256).
 constexpr int kColRowsPerLane = 4;
 
-// |col|^2 as every zeroton test computes it: per-lane sums over the lane's
-// rows in order with explicit FMAs, an xor butterfly, then sqrt and square
-// (np.linalg.norm(col) ** 2). The assembler's energies, the peeler's screen
-// and its per-pass detection all call this, so they classify a column alike.
+// |col|^2 as every zeroton test computes it: per-lane sums of re^2 + im^2 over
+// the lane's rows in order with explicit FMAs, then one xor butterfly (the
+// reference's np.linalg.norm(col) ** 2 up to summation order and the rounding
+// of its square root and square, a few ulp). The assembler's energies, the
+// peeler's screen and its per-pass detection all call this, so they classify
+// a column alike.
 template <int N> __device__ __forceinline__ double bin_energy(const double2 (&u)[N]) {
-    double sre = 0.0, sim = 0.0;
+    double s = 0.0;
 #pragma unroll
     for (int r = 0; r < N; ++r) {
-        sre = __fma_rn(u[r].x, u[r].x, sre);
-        sim = __fma_rn(u[r].y, u[r].y, sim);
+        s = __fma_rn(u[r].x, u[r].x, s);
+        s = __fma_rn(u[r].y, u[r].y, s);
     }
-    sre = wave_sum(sre);
-    sim = wave_sum(sim);
-    const double e = sqrt(__dadd_rn(sre, sim));
-    return __dmul_rn(e, e);
+    return wave_sum(s);
 }
 
 // LDS byte address of a __shared__ object
