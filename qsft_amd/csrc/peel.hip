@@ -891,10 +891,11 @@ __device__ __forceinline__ void detect_bins(const qsft_peel_desc& d, int it, int
 // singletons in LDS. Then one lane per buffered singleton registers it in the
 // hash table, so the atomics of up to 16 registrations are in flight
 // together, and the pass counters take one atomic per wave. Four waves per
-// SIMD fit without scratch (at most 128 VGPRs): the FP64 atan2 rules, whose
-// registers would otherwise be reserved across the whole loop, are out of line.
+// SIMD fit without scratch for two rows per lane (at most 128 VGPRs; three for
+// four rows): the FP64 atan2 rules, whose registers would otherwise be
+// reserved across the whole loop, are out of line.
 template <int NR>
-__global__ __launch_bounds__(kBinThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_peel_detect(const qsft_peel_desc* __restrict__ descs,
+__global__ __launch_bounds__(kBinThreads) __attribute__((amdgpu_waves_per_eu(NR == 2 ? 4 : 3))) void k_peel_detect(const qsft_peel_desc* __restrict__ descs,
                                                              const int32_t* __restrict__ pre, int n_inst, int it) {
     __shared__ double2 tab[QSFT_MAX_Q];
     __shared__ DetectWaveLds<NR> wl[kWaves];
