@@ -81,7 +81,8 @@ class QSFT:
         t0 = time.time()
         batch = kernels.PeelBatch([dict(q=q, n=n, b=b, C=C, P=P, p1=signal.get_source_parity(), method=method,
                                         cutoff=cutoff, Mt=Mt, D=Dd, U=U)], max_iter=MAX_ITER)
-        batch.run(host_loop=self.host_loop)
+        # passes 3.. as one launch (kernels.PeelBatch.run): the same results, no launch per pass
+        batch.run(host_loop=self.host_loop, fuse_from=None if self.host_loop else 3)
         k, vals, _, info = batch.results()[0]
         peeling_time = time.time() - t0
         if timing_verbose:
