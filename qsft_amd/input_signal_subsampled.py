@@ -21,6 +21,7 @@ or "fp32") selects the sample/sub-transform precision.
 import random
 import time
 from math import floor
+from pathlib import Path
 
 import numpy as np
 import torch
@@ -64,8 +65,26 @@ class SubsampledSignal(Signal):
             self._subsample_qsft()
 
     def _set_Ms_and_Ds_qsft(self):
-        """M and D from query_args (qsft/input_signal_subsampled.py:92-105; no disk cache)."""
+        """M and D from query_args, or from the folder cache when it holds them
+        (qsft/input_signal_subsampled.py:92-105: then no M/D draws are taken
+        from the global RNG). The cache is this package's own format: numpy
+        archives read with allow_pickle=False (the reference pickles; its
+        cache files are not read)."""
+        path = None
+        if self.foldername:
+            Path(self.foldername).mkdir(parents=True, exist_ok=True)
+            path = Path(self.foldername) / "Ms_and_Ds.npz"
+            if path.is_file():
+                with np.load(path, allow_pickle=False) as z:
+                    Ms, Ds = z["Ms"], z["Ds"]
+                if Ms.ndim == 3 and Ms.shape[1] == self.n and Ds.ndim == 4 and Ds.shape[3] == self.n:
+                    self.Ms = [Ms[i] for i in range(Ms.shape[0])]
+                    self.Ds = [[Ds[i, j] for j in range(Ds.shape[1])] for i in range(Ds.shape[0])]
+                    return
         self.Ms, self.Ds = get_Ms_and_Ds(self.n, self.q, **self.query_args)
+        if path is not None:
+            np.savez_compressed(path, Ms=np.stack([np.asarray(m) for m in self.Ms]),
+                                Ds=np.stack([np.stack([np.asarray(d) for d in row]) for row in self.Ds]))
 
     # ------------------------------------------------------------ sampling
 
@@ -89,13 +108,31 @@ class SubsampledSignal(Signal):
         self._Dall = dv.D_tensor([np.vstack(self.Ds[i]) for i in range(C)], n_pad, dev)
 
     def _subsample_qsft(self):
-        """input_signal_subsampled.py:107-155: samples, then sub-DFTs per b in all_bs."""
+        """input_signal_subsampled.py:107-155: samples, then sub-DFTs per b in all_bs.
+        With a folder, the transforms (every (group, repeat) and every b in
+        all_bs) are loaded when all are cached, else the samples when all are
+        cached; whatever is computed is written back (samples/M{i}_D{j}.npz,
+        transforms/U{i}_{j}.npz)."""
         self._upload_queries()
         q, b = self.q, self.b
         C, R, p1 = self._layout()
-        x = self._device_samples()
+        folder = Path(self.foldername) if self.foldername else None
+        if folder is not None:
+            (folder / "samples").mkdir(parents=True, exist_ok=True)
+            (folder / "transforms").mkdir(parents=True, exist_ok=True)
+            if self._load_cached_transforms(folder):
+                return
+        x = self._load_cached_samples(folder) if folder is not None else None
         if x is None:
-            x = self._callback_samples()
+            x = self._device_samples()
+            if x is None:
+                x = self._callback_samples()
+            if folder is not None:
+                h = x.cpu().numpy()
+                for i in range(C):
+                    for j in range(R):
+                        np.savez_compressed(folder / "samples" / f"M{i}_D{j}.npz",
+                                            samples=h[i, j * p1:(j + 1) * p1])
         rows = C * R * p1
         xr = x.reshape(rows, q ** b)
         self._U = {}
@@ -111,6 +148,51 @@ class SubsampledSignal(Signal):
                 for j in range(R):
                     self.transformTimes[i][j][bb] = per
         self._Us_host = None
+        if folder is not None:
+            hs = {bb: self._U[bb].cpu().numpy() for bb in self.all_bs}
+            for i in range(C):
+                for j in range(R):
+                    arrs = {f"U{bb}": hs[bb][i, j * p1:(j + 1) * p1] for bb in self.all_bs}
+                    arrs.update({f"t{bb}": np.float64(self.transformTimes[i][j][bb]) for bb in self.all_bs})
+                    np.savez_compressed(folder / "transforms" / f"U{i}_{j}.npz", **arrs)
+
+    def _load_cached_samples(self, folder):
+        """Samples [C][R*p1][q^b] from the folder when every (group, repeat) is there, else None."""
+        C, R, p1 = self._layout()
+        files = [[folder / "samples" / f"M{i}_D{j}.npz" for j in range(R)] for i in range(C)]
+        if not all(f.is_file() for row in files for f in row):
+            return None
+        h = np.zeros((C, R * p1, self.q ** self.b), dtype=np.complex128)
+        for i in range(C):
+            for j in range(R):
+                with np.load(files[i][j], allow_pickle=False) as z:
+                    if "samples" not in z.files or z["samples"].shape != (p1, self.q ** self.b):
+                        return None                                # written for other sizes: recompute
+                    h[i, j * p1:(j + 1) * p1] = z["samples"]
+        return torch.from_numpy(h).to(self._dev).to(dv.PRECISIONS[self.precision][1])
+
+    def _load_cached_transforms(self, folder):
+        """Sub-DFTs of every b in all_bs and their times from the folder; False unless all are there."""
+        C, R, p1 = self._layout()
+        files = [[folder / "transforms" / f"U{i}_{j}.npz" for j in range(R)] for i in range(C)]
+        if not all(f.is_file() for row in files for f in row):
+            return False
+        hs = {bb: np.zeros((C, R * p1, self.q ** bb), dtype=np.complex128) for bb in self.all_bs}
+        times = [[{} for _ in range(R)] for _ in range(C)]
+        for i in range(C):
+            for j in range(R):
+                with np.load(files[i][j], allow_pickle=False) as z:
+                    if not all(f"U{bb}" in z.files and z[f"U{bb}"].shape == (p1, self.q ** bb)
+                               for bb in self.all_bs):
+                        return False                               # other b or sizes: recompute
+                    for bb in self.all_bs:
+                        hs[bb][i, j * p1:(j + 1) * p1] = z[f"U{bb}"]
+                        times[i][j][bb] = float(z[f"t{bb}"])
+        dt = dv.PRECISIONS[self.precision][1]
+        self._U = {bb: torch.from_numpy(h).to(self._dev).to(dt) for bb, h in hs.items()}
+        self.transformTimes = times
+        self._Us_host = None
+        return True
 
     def _callback_samples(self):
         """Generic plugin route: decimal query indices -> self.subsample()."""

@@ -363,3 +363,55 @@ def test_synthetic_helper_heldout_nmse(tmp_path):
     assert helper.test_model("qsft", beta={}) == 1
     with pytest.raises(NotImplementedError):
         helper.compute_model("lasso", {})
+
+
+@pytest.mark.gpu
+def test_folder_cache_of_delays_samples_and_transforms(tmp_path):
+    """The `folder` cache (qsft/input_signal_subsampled.py:92-155): the first
+    construction writes M/D, samples and every b's sub-DFTs and takes the same
+    RNG draws as no cache; a second one loads M/D (no M/D draws) and the
+    transforms (bit-identical, same decode); with only samples cached the
+    sub-DFTs are recomputed from them, bit-identical again."""
+    import shutil
+
+    from qsft_amd.qsft import QSFT
+    from qsft_amd.synthetic_signal import get_random_subsampled_signal
+
+    q, n, S, b, C, R, sd = 4, 10, 30, 4, 3, 2, 0.5
+    qa = dict(query_method="complex", num_subsample=C, b=b, all_bs=[b - 1, b], delays_method_source="identity",
+              delays_method_channel="nso", num_repeat=R, subsampling_method="qsft")
+
+    def make(folder):
+        np.random.seed(7)
+        sig = get_random_subsampled_signal(n=n, q=q, noise_sd=sd, sparsity=S, a_min=1, a_max=10,
+                                           query_args=dict(qa), verbose=False, folder=folder)
+        return sig, np.random.rand()
+
+    def decode(sig):
+        np.random.seed(99)
+        return QSFT(num_subsample=C, num_repeat=R, reconstruct_method_source="identity",
+                    reconstruct_method_channel="nso", b=b).transform(sig)
+
+    def same_us(a, b_):
+        for bb in qa["all_bs"]:
+            for i in range(C):
+                for j in range(R):
+                    assert np.array_equal(np.asarray(a.Us[i][j][bb]), np.asarray(b_.Us[i][j][bb]))
+
+    plain, r_plain = make(None)
+    first, r_first = make(str(tmp_path))
+    assert r_first == r_plain                                          # writing the cache changes no draw
+    assert (tmp_path / "Ms_and_Ds.npz").is_file()
+    assert all((tmp_path / "samples" / f"M{i}_D{j}.npz").is_file() for i in range(C) for j in range(R))
+    assert all((tmp_path / "transforms" / f"U{i}_{j}.npz").is_file() for i in range(C) for j in range(R))
+    loaded, r_loaded = make(str(tmp_path))
+    assert r_loaded != r_plain                                         # M/D came from the cache
+    assert all(np.array_equal(a, c) for a, c in zip(loaded.Ms, plain.Ms))
+    same_us(loaded, plain)
+    want = decode(plain)
+    got = decode(loaded)
+    assert list(got) == list(want) and all(got[k] == want[k] for k in want)
+    shutil.rmtree(tmp_path / "transforms")
+    resampled, _ = make(str(tmp_path))                                 # samples cached, transforms recomputed
+    same_us(resampled, plain)
+    assert all((tmp_path / "transforms" / f"U{i}_{j}.npz").is_file() for i in range(C) for j in range(R))
