@@ -139,7 +139,7 @@ def test_short_pass_launch_is_finished_on_fetch():
     assert short.passes <= short.max_iter and short.peel.last_iters >= 2
 
 
-@pytest.mark.parametrize("case", ["k1_mixed_snr", "p_over_128", "k1_to_cap"])
+@pytest.mark.parametrize("case", ["k1_mixed_snr", "p_over_128", "k1_to_cap", "tiny_and_odd_q"])
 def test_fused_tail_passes_match_per_pass_launches(case):
     """Passes fuse_from .. 14 as one kernel (one workgroup per instance looping
     over the passes, qsft_peel_passes) give what one launch pair per pass
@@ -151,6 +151,10 @@ def test_fused_tail_passes_match_per_pass_launches(case):
         specs = [E.random_spec(4, 50, 1000, 6, 3, 2, snr, rng=rng) for snr in (0.0, 3.0, 5.0, 10.0, 10.0, 20.0, None)]
     elif case == "p_over_128":
         specs = [E.random_spec(4, 100, 400, 5, 3, 2, snr, rng=rng) for snr in (3.0, 10.0, None)]
+    elif case == "tiny_and_odd_q":
+        # fewer bins than one 64-bin group (q = 2, b = 3: 24 bins), a non-power-of-two q, noiseless loops
+        specs = [E.random_spec(2, 8, 6, 3, 3, 2, None, rng=rng), E.random_spec(2, 12, 10, 3, 3, 2, 10.0, rng=rng),
+                 E.random_spec(3, 10, 40, 4, 3, 2, 15.0, rng=rng), E.random_spec(3, 10, 40, 4, 3, 2, None, rng=rng)]
     else:
         # the two batches of 64 K1 transforms at 10 dB that bench.py serves, at
         # its noise draws: one of them leaves a loop that still finds
