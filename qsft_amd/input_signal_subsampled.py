@@ -77,7 +77,14 @@ class SubsampledSignal(Signal):
             if path.is_file():
                 with np.load(path, allow_pickle=False) as z:
                     Ms, Ds = z["Ms"], z["Ds"]
-                if Ms.ndim == 3 and Ms.shape[1] == self.n and Ds.ndim == 4 and Ds.shape[3] == self.n:
+                # reuse only a cache that covers this request: enough groups,
+                # columns and repeats, and digits of this q
+                fits = (Ms.ndim == 3 and Ds.ndim == 4 and Ms.shape[1] == self.n and Ds.shape[3] == self.n
+                        and Ms.shape[0] >= self.num_subsample and Ms.shape[2] >= self.b
+                        and Ds.shape[0] == Ms.shape[0] and Ds.shape[1] >= self.num_repeat
+                        and Ms.size > 0 and Ds.size > 0
+                        and 0 <= Ms.min() and Ms.max() < self.q and 0 <= Ds.min() and Ds.max() < self.q)
+                if fits:
                     self.Ms = [Ms[i] for i in range(Ms.shape[0])]
                     self.Ds = [[Ds[i, j] for j in range(Ds.shape[1])] for i in range(Ds.shape[0])]
                     return
