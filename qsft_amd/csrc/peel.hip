@@ -700,11 +700,11 @@ __device__ __forceinline__ void detect_bins(const qsft_peel_desc& d, int it, int
 
         while (t >= (c + 1) * Q) ++c;
         const int j = t - c * Q;
-        // (a sparse U needs the assembler's energies, so P <= 128: NR = 2)
-        const bool rebuilt = NR == 2 && ((unwritten >> cur) & 1);  // wave-uniform
-        if constexpr (NR == 2) {
-            if (rebuilt) rebuild_col(d, c, j, u, lane);            // (its prefetched copy is not U)
-        }
+        // A sparse U needs the assembler's energies, so this instance has
+        // P <= 128, but NR follows the batch's largest P: NR = 4 kernels
+        // rebuild skipped columns too (rows past P stay zero).
+        const bool rebuilt = (unwritten >> cur) & 1;               // wave-uniform
+        if (rebuilt) rebuild_col(d, c, j, u, lane);                // (its prefetched copy is not U)
         // balls queued on this bin by the previous pass
         apply_pending<PERSIST>(d, t, c, it - 1, pd, u, W.pslot[cslot].krow, kk, tab, mq, lane, offbuf, offs_staged);
         int mval = 0;                                              // mark to store (0: none)
@@ -773,9 +773,24 @@ __device__ __forceinline__ void detect_bins(const qsft_peel_desc& d, int it, int
             rho_x = rr * inv_p;                                    // within 1 ulp of rr / P
             rho_y = ri * inv_p;
             // residual energy |col - rho sig|^2 = |col|^2 - P |rho|^2 (|sig_p| = 1 and
-            // sig^H col = P rho); rounding error ~1e-16 |col|^2, far below any cutoff
-            // that separates a singleton from a multiton
-            const double res = energy - (double)P * (rho_x * rho_x + rho_y * rho_y);
+            // sig^H col = P rho); its rounding error is ~1e-15 |col|^2. Where that
+            // could exceed 1e-3 of the threshold (|col|^2 > 1e12 thr: large
+            // amplitudes against the noiseless cutoff 1e-9 P) the residual is
+            // summed explicitly over the rows instead, as the reference forms it.
+            double res = energy - (double)P * (rho_x * rho_x + rho_y * rho_y);
+            if (energy > 1e12 * thr) {                             // wave-uniform
+                double s = 0.0;
+#pragma unroll
+                for (int r = 0; r < NR; ++r) {
+                    const int p = lane + 64 * r;
+                    if (p < P) {
+                        const double dx = u[r].x - (rho_x * w[r].x - rho_y * w[r].y);
+                        const double dy = u[r].y - (rho_x * w[r].y + rho_y * w[r].x);
+                        s += dx * dx + dy * dy;
+                    }
+                }
+                res = wave_sum(s);
+            }
 
             // --- bin match: the bin of k in group c, sum over digits dg (lane dg)
             // of (<M_c^T row dg, k> mod q) q^(b - 1 - dg), equals j

@@ -227,6 +227,41 @@ def test_sparse_u_matches_dense_u(everything):
         print(f"instance {i}: written {int(written.sum())} of {written.size}, rebuilt below the cutoff {rebuilt}")
 
 
+def test_sparse_u_beside_an_instance_over_128_rows():
+    """A batch whose largest instance has P > 128 runs the four-rows-per-lane
+    peeler for every instance, while its instances with P <= 128 keep a
+    sparse working transform. Columns the assembler skipped must then be
+    rebuilt by that peeler too: every instance gives the rows it gives alone
+    (two-rows-per-lane peeler) and the rows of the same batch with every
+    column written."""
+    rng = np.random.default_rng(17)
+    specs = [E.random_spec(2, 70, 60, 7, 3, 2, 12.0, rng=rng)]                  # P = 142: dense U
+    specs += [E.random_spec(4, 50, 1000, 6, 3, 2, snr, rng=rng) for snr in (0.0, 3.0, 5.0)]
+    specs += [E.random_spec(3, 20, 90, 4, 3, 2, 2.0, rng=rng), E.random_spec(4, 30, 100, 4, 3, 2, None, rng=rng)]
+    print("sparse U next to P = 142", flush=True)
+    mixed = E.Batch(specs, precision="fp32", sparse_u=True)
+    dense = E.Batch(specs, precision="fp32", sparse_u=False)
+    assert mixed.peel.max_p > 128
+    assert all(d.flags & 8 for d in mixed.peel.descs[1:]) and not mixed.peel.descs[0].flags & 8
+    mixed.launch(seed_offset=4)
+    dense.launch(seed_offset=4)
+    got = mixed.results()
+    for a, b in zip(got, dense.results()):
+        _identical(a, b)
+    rebuilt = 0
+    for i, sp in enumerate(specs):
+        alone = E.Batch([sp], precision="fp32", sparse_u=True)
+        alone.launch(seed_offset=4)
+        (k, v, _, _), = alone.results()
+        _same((got[i][0], got[i][1]), (k, v), rtol=1e-12)
+        if i > 0:
+            written = _written_flags(mixed, i)
+            energy = mixed.bufs[i]["energy"].cpu().numpy().ravel()
+            rebuilt += int(np.sum((written == 1) & (energy <= sp.threshold() * sp.P)))
+    print("columns rebuilt below the cutoff:", rebuilt, flush=True)
+    assert rebuilt > 0                                                          # the case was exercised
+
+
 def test_k2_served_fp32_matches_fp64_path():
     """q = 20, n = 16, S = 2000, b = 3 (8000 bins, radix-20 DFT), NSO R = 2, 20 dB:
     the served FP32 path (matrix-core evaluation, FP32 DFT, FP64 peel) against
