@@ -4,8 +4,10 @@ through the device peeler.
 
 Bars: recovered support (set of k) bit-exact; coefficients within 1e-9
 relative of the oracle in FP64 mode. Full-size configurations, which the
-oracle cannot run in seconds, are checked through size-independent
-properties: noiseless support == planted support, NMSE <= 1e-3 at 10 dB.
+oracle cannot run in seconds, are checked against the reference's golden
+vectors (tests/test_gpu_golden.py) and through size-independent properties:
+noiseless support == planted support; at 10 dB an NMSE at the reference
+decoder's own level on the same inputs.
 """
 import numpy as np
 import pytest
@@ -238,7 +240,7 @@ def test_generic_subsample_callback_path():
     samples from the callback, sub-transforms on the device."""
     from qsft_amd.input_signal_subsampled import SubsampledSignal
 
-    q, n, S, b = 3, 8, 12, 2
+    q, n, S, b = 3, 8, 12, 3
     rng = np.random.default_rng(31)
     locq = rng.integers(0, q, size=(n, S))
     a = rng.normal(size=S) + 1j * rng.normal(size=S)
@@ -258,12 +260,25 @@ def test_generic_subsample_callback_path():
     for i in range(3):
         cf = O.closed_form_U(sig.Ms[i], sig.Ds[i][0], locq, a, q)
         np.testing.assert_allclose(np.array(sig.Us[i][0][b]), cf, rtol=0, atol=1e-12)
+    np.random.seed(7)
     gw = QSFT(num_subsample=3, num_repeat=1, reconstruct_method_source="identity",
               reconstruct_method_channel="identity", b=b).transform(sig)
+    # the oracle decodes the same stored transforms after the same group and
+    # repeat permutation (the base get_MDU adds no noise)
+    np.random.seed(7)
+    sub = np.random.choice(3, 3, replace=False)
+    rep = np.random.choice(1, 1, replace=False)
+    Ms = [sig.Ms[i][:, :b] for i in sub]
+    Ds = [np.vstack([sig.Ds[i][j] for j in rep]) for i in sub]
+    Us = np.array([np.vstack([O.closed_form_U(sig.Ms[i], sig.Ds[i][j], locq, a, q) for j in rep]) for i in sub])
+    _, ref, _ = O.peel_decode(Ms, Ds, Us, q, n, 1e-9, method_channel="identity")
+    assert_same_result(gw, ref)
     truth = {}
     for s in range(S):
         truth[tuple(locq[:, s])] = truth.get(tuple(locq[:, s]), 0) + a[s]
-    assert set(gw) <= set(truth)
+    assert set(gw) <= set(truth) and len(gw) >= len(truth) // 2, (len(gw), len(truth))
+    for k, v in gw.items():
+        assert abs(v - truth[k]) <= 1e-9 * abs(truth[k])
 
 
 # ------------------------------------------------- full-size configurations
@@ -285,19 +300,64 @@ def test_k1_noiseless_fp32_support_exact():
     assert err < 1e-5
 
 
-def test_k1_noisy_fp32_nmse():
-    q, n, S, b = 4, 50, 1000, 6
+def test_k1_noisy_fp32_nmse_at_the_references_level():
+    """q=4, n=50, S=1000, b=6, NSO R=2 at 10 dB, on the reference's own inputs
+    (tests/golden/k1_10db.npz): the served FP32 path with device noise, over 8
+    noise draws, has a median NMSE within 1.3x of the reference decoder's
+    NMSE on those inputs (4.9e-3: nso1 with R = 2 misses small coefficients
+    at this SNR, so 1e-3 is not reached by the reference either), and the
+    drop-in FP32 route with device noise lands in the same range."""
+    import golden_data as G
+    from qsft_amd import engine as E
+    fx = G.load("k1_10db")
+    q, n, b, C, R = fx["q"], fx["n"], fx["b"], fx["C"], fx["R"]
+    truth = {tuple(int(x) for x in k): v for k, v in zip(fx["signal_w_keys"], fx["signal_w_vals"])}
+    est = {tuple(int(x) for x in k): v for k, v in zip(fx["gwht_keys"], fx["gwht_vals"])}
+    ref_nmse = O.nmse(est, truth)
+    specs = [E.TransformSpec(q=q, n=n, b=b, C=C, R=R, locq=fx["locq"].astype(np.int64),
+                             strengths=fx["signal_w_vals"], Ms=[m.astype(np.int64) for m in fx["Ms"]],
+                             D=np.vstack(fx["Ds"][0]).astype(np.int64), noise_sd=fx["noise_sd"], seed=s)
+             for s in range(8)]
+    bt = E.Batch(specs, precision="fp32")
+    bt.launch(seed_offset=1)
+    nm = [E.nmse(k, v, sp) for (k, v, _, _), sp in zip(bt.results(), specs)]
+    print(f"reference NMSE {ref_nmse:.3e}; served FP32 median {np.median(nm):.3e} "
+          f"(min {min(nm):.3e}, max {max(nm):.3e})", flush=True)
+    assert 3e-3 < ref_nmse < 8e-3
+    assert np.median(nm) <= 1.3 * ref_nmse
     qa = dict(query_method="complex", num_subsample=3, b=b, delays_method_source="identity",
               delays_method_channel="nso", num_repeat=2, subsampling_method="qsft")
     np.random.seed(102)
-    sig = get_random_subsampled_signal(n=n, q=q, noise_sd=0.0, sparsity=S, a_min=1, a_max=10, query_args=qa,
+    sig = get_random_subsampled_signal(n=n, q=q, noise_sd=0.0, sparsity=1000, a_min=1, a_max=10, query_args=qa,
                                        verbose=False, precision="fp32", noise_mode="device")
     sig.noise_sd = float(np.sqrt(np.sum(np.abs(sig.strengths) ** 2) / 10 ** (10 / 10)))   # 10 dB
     gw = QSFT(num_subsample=3, num_repeat=2, reconstruct_method_source="identity",
               reconstruct_method_channel="nso", b=b).transform(sig)
-    nm = O.nmse(gw, sig.signal_w)
-    print(f"K1 10 dB NMSE {nm:.3e}, recovered {len(gw)}", flush=True)
-    assert nm <= 1e-2
+    nm_dropin = O.nmse(gw, sig.signal_w)
+    print(f"drop-in FP32 device noise: NMSE {nm_dropin:.3e}, recovered {len(gw)}", flush=True)
+    assert nm_dropin <= 2.0 * ref_nmse
+
+
+@pytest.mark.parametrize("channel,R", [("identity", 1), ("nso", 2)])
+def test_large_amplitudes_noiseless_match_oracle(channel, R):
+    """Strengths of 5e3..2e4 against the noiseless cutoff (1e-9 per row): a
+    residual formed as |col|^2 - P|rho|^2 would carry a rounding error above
+    the cutoff and call true singletons multitons. Same seed as the oracle:
+    the same support (the planted one), order and values."""
+    q, n, S, b, C = 4, 16, 120, 4, 3
+    qa = dict(query_method="complex", num_subsample=C, b=b, delays_method_source="identity",
+              delays_method_channel=channel, num_repeat=R, subsampling_method="qsft")
+    print(f"large amplitudes, {channel}", flush=True)
+    np.random.seed(31)
+    ref_sig = O.get_random_subsampled_signal(n, q, 0.0, S, 5e3, 2e4, dict(qa))
+    ref = O.qsft_transform(ref_sig, C, R, b, "identity", channel)
+    np.random.seed(31)
+    sig = get_random_subsampled_signal(n=n, q=q, noise_sd=0.0, sparsity=S, a_min=5e3, a_max=2e4,
+                                       query_args=dict(qa), verbose=False)
+    got = QSFT(num_subsample=C, num_repeat=R, reconstruct_method_source="identity",
+               reconstruct_method_channel=channel, b=b).transform(sig)
+    assert set(ref) == set(sig.signal_w)
+    assert_same_result(got, ref)
 
 
 @pytest.mark.parametrize("channel", ["nso", "identity", "random"])
