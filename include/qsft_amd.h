@@ -362,6 +362,32 @@ size_t qsft_peel_pack_header_bytes(int n_inst);
 int qsft_peel_pack(const qsft_peel_desc* dev_descs, int n_inst, void* dst, int64_t cap_rows, int row_stride,
                    void* stream);
 
+/* ------------------------------------------------------------------------
+ * No counterpart in the reference. For callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops and harnesses that
+ * forbid hipMalloc): one instance's whole peeling loop, the same work and
+ * results as qsft_peel_bind + qsft_peel_run + qsft_peel_pack(n_inst = 1),
+ * with the descriptor, the block prefixes and the peel workspace held in a
+ * static device array of the library instead of caller memory. The caller
+ * passes only the inputs (Mt, D as for qsft_peel_bind, U bin-major FP64,
+ * modified in place) and the packed output `dst` (header + rows, as
+ * qsft_peel_pack writes them). flags: QSFT_PEEL_SHARED_D / UNIT_D /
+ * UNIT_D_NEG (no sparse U, no assembler energies).
+ * Limit: qsft_peel_workspace_bytes(q, n, b, C, P, max_iter, 0) must be at most
+ * QSFT_NOALLOC_SCRATCH_BYTES - 768 (64 MiB scratch: covers C = 3 groups of
+ * q^b = 4096 bins at n = 50 or 8000 bins at n = 16; larger shapes get
+ * QSFT_ERR_ARG). Each GPU holds its own scratch copy from the first call on
+ * it until the process ends. Every call on one GPU reuses that GPU's copy
+ * (the loop's init kernel clears what must start empty), so calls that
+ * overlap on one GPU -- on different streams or threads -- race: serialize
+ * them, e.g. on one stream.
+ * ------------------------------------------------------------------------ */
+#define QSFT_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+size_t qsft_noalloc_scratch_bytes(void);
+int qsft_peel_transform_noalloc(int q, int n, int b, int C, int P, int p1, int method, int max_iter,
+                                double cutoff, int flags, const int8_t* Mt, const int8_t* D, double* U,
+                                int n_pad, void* dst, int64_t cap_rows, int row_stride, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

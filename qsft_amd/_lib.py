@@ -118,6 +118,10 @@ SIGNATURES = {
     "qsft_peel_finalize": (ctypes.c_int, [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp]),
     "qsft_peel_pack_header_bytes": (ctypes.c_size_t, [ctypes.c_int]),
     "qsft_peel_pack": (ctypes.c_int, [vp, ctypes.c_int, vp, ctypes.c_int64, ctypes.c_int, vp]),
+    "qsft_noalloc_scratch_bytes": (ctypes.c_size_t, []),
+    "qsft_peel_transform_noalloc": (ctypes.c_int, [ctypes.c_int] * 8 + [ctypes.c_double, ctypes.c_int, vp, vp, vp,
+                                                                        ctypes.c_int, vp, ctypes.c_int64,
+                                                                        ctypes.c_int, vp]),
 }
 
 

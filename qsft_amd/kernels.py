@@ -399,6 +399,52 @@ def unit_delay_sign(D, q, n, p1):
     return 0
 
 
+def peel_flags(it):
+    """QSFT_PEEL_SHARED_D / UNIT_D / UNIT_D_NEG bits of a peel item (see PeelBatch)."""
+    C, q, n = it["C"], it["q"], it["n"]
+    shared = it.get("shared_d")
+    if shared is None:
+        Dg = it["D"]
+        shared = all(torch.equal(Dg[0], Dg[c]) for c in range(1, C))
+    flags = L.QSFT_PEEL_SHARED_D if shared else 0
+    unit = it.get("unit_d")
+    if unit is None:
+        unit = unit_delay_sign(it["D"][:1] if shared else it["D"], q, n, it["p1"])
+    if unit:
+        flags |= L.QSFT_PEEL_UNIT_D | (L.QSFT_PEEL_UNIT_D_NEG if unit < 0 else 0)
+    return flags
+
+
+def peel_noalloc(it, dst, cap_rows, row_stride, max_iter=15):
+    """One instance's whole peeling loop through qsft_peel_transform_noalloc:
+    the descriptor and workspace live in the library's static scratch, so
+    nothing is allocated; `dst` (device or pinned host memory) receives the
+    packed header and rows exactly as PeelBatch's pack does for one instance.
+    `it` is a PeelBatch item without energy / sparse entries."""
+    if it.get("energy") is not None or it.get("sparse") is not None:
+        raise ValueError("the no-allocation peel reads U itself (no assembler energies, no sparse U)")
+    L.check(lib.qsft_peel_transform_noalloc(
+        it["q"], it["n"], it["b"], it["C"], it["P"], it["p1"], it["method"], max_iter, float(it["cutoff"]),
+        peel_flags(it), _ptr(it["Mt"]), _ptr(it["D"]), _ptr(it["U"]), it["Mt"].shape[-1], _vp(dst.data_ptr()),
+        int(cap_rows), int(row_stride), stream_handle()), "qsft_peel_transform_noalloc")
+
+
+def unpack_rows(hdr, rows, n_inst, ns, n_pads):
+    """Per instance (k int64 [m][n], values complex128 [m], order uint64 [m]),
+    sorted by first detection, from a packed header (int64) and rows (uint8
+    [total][row_stride])."""
+    counts, offs = hdr[:n_inst], hdr[n_inst:2 * n_inst]
+    out = []
+    for i in range(n_inst):
+        r = rows[offs[i]:offs[i] + counts[i]]
+        k = r[:, :ns[i]].astype(np.int64)
+        v = r[:, n_pads[i]:n_pads[i] + 16].copy().view(np.float64).reshape(-1, 2)
+        order = r[:, n_pads[i] + 16:n_pads[i] + 24].copy().view(np.uint64).reshape(-1)
+        srt = np.argsort(order, kind="stable")
+        out.append((k[srt], (v[:, 0] + 1j * v[:, 1])[srt], order[srt]))
+    return out
+
+
 class PeelBatch:
     """Bound peel descriptors + workspaces for a list of instances.
 
@@ -437,16 +483,7 @@ class PeelBatch:
             L.check(lib.qsft_peel_bind(ctypes.byref(d), q, n, b, C, P, it["p1"], it["method"], max_iter,
                                        float(it["cutoff"]), 0, _ptr(it["Mt"]), _ptr(it["D"]), _ptr(it["U"]),
                                        _ptr(ws), nbytes, n_pad), "qsft_peel_bind")
-            shared = it.get("shared_d")
-            if shared is None:
-                Dg = it["D"]
-                shared = all(torch.equal(Dg[0], Dg[c]) for c in range(1, C))
-            d.flags = L.QSFT_PEEL_SHARED_D if shared else 0
-            unit = it.get("unit_d")
-            if unit is None:
-                unit = unit_delay_sign(it["D"][:1] if shared else it["D"], q, n, it["p1"])
-            if unit:
-                d.flags |= L.QSFT_PEEL_UNIT_D | (L.QSFT_PEEL_UNIT_D_NEG if unit < 0 else 0)
+            d.flags = peel_flags(it)
             if it.get("energy") is not None:
                 d.energy = _ptr(it["energy"])
             sp = it.get("sparse")
