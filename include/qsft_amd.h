@@ -49,7 +49,7 @@ enum { QSFT_DEVERR_TABLE_FULL = 1, QSFT_DEVERR_TAG_COLLISION = 2, QSFT_DEVERR_OU
 const char* qsft_last_error(void);
 int qsft_abi_version(void);
 /* sizeof() of the descriptor structs, for FFI layout checks:
- * which = 0 eval, 1 eval_points, 2 dft, 3 assemble, 4 peel, 5 eval_mfma. */
+ * which = 0 eval, 1 eval_points, 2 dft, 3 assemble, 4 peel, 5 eval_mfma, 6 request, 7 batch sizes. */
 size_t qsft_sizeof(int which);
 
 /* ------------------------------------------------------------------------
@@ -361,6 +361,78 @@ int qsft_peel_finalize(const qsft_peel_desc* dev_descs, const int32_t* dev_pre_s
 size_t qsft_peel_pack_header_bytes(int n_inst);
 int qsft_peel_pack(const qsft_peel_desc* dev_descs, int n_inst, void* dst, int64_t cap_rows, int row_stride,
                    void* stream);
+
+/* The same rows, each instance's in first-detection order (the reference's
+ * dict insertion order, qsft/qsft.py:247-255): a row's position is its
+ * instance offset plus its rank by the order word, so readers need no sort. */
+int qsft_peel_pack_ordered(const qsft_peel_desc* d, int n_inst, void* dst, int64_t cap_rows, int row_stride,
+                           void* stream);
+
+/* ------------------------------------------------------------------------
+ * Batch runtime: whole transforms of many synthetic requests, planned and
+ * enqueued natively. Replaces, per request, what one
+ * QSFT(**qsft_args).transform(get_random_subsampled_signal(...)) call does:
+ * SubsampledSignal._subsample_qsft (qsft/input_signal_subsampled.py:107-155,
+ * with SyntheticSubsampledSignal.subsample, synt_exp/synt_src/synthetic_signal.py:97-118),
+ * get_MDU + noise (qsft/input_signal_subsampled.py:225-262,
+ * synt_exp/synt_src/synthetic_signal.py:120-130) and the peeling loop with its
+ * result dict (qsft/qsft.py:98-281) -- for all groups, repeats and b of one
+ * request as the reference's loops do one by one, and for all requests of a
+ * batch in the same launches (the reference serves one request per call).
+ * Samples and sub-DFTs in FP32 on the matrix cores, working transform, peeling
+ * and coefficients in FP64; every group shares the request's delay rows.
+ *
+ * Usage: qsft_batch_create (sizes) -> per request qsft_batch_pack_input into a
+ * host staging arena -> qsft_batch_bind to device arenas of the returned sizes,
+ * a host plan buffer and a device plan buffer (all 256-byte aligned) and the
+ * packed output (pinned host memory or device memory; the qsft_peel_pack
+ * format, rows ordered as qsft_peel_pack_ordered writes them) ->
+ * qsft_batch_launch (upload bit 0: copy the staged inputs, bit 1: the plan;
+ * later launches of the same batch may skip both and only draw fresh noise
+ * with a new seed_offset) -> qsft_batch_done / qsft_batch_wait -> read `out`.
+ * A launch first waits for the batch's previous launch to finish. With a
+ * tail stream, finalize and pack run there after the peeling passes, so the
+ * next batch's work on `stream` overlaps them.
+ * ------------------------------------------------------------------------ */
+typedef struct qsft_request {
+    int32_t q, n, b, C, R, S;   /* P = R (n + 1) delay rows, shared by the C groups            */
+    int32_t method;             /* QSFT_DETECT_NSO1, or QSFT_DETECT_NOISELESS with R = 1        */
+    int32_t _pad0;
+    double cutoff;              /* zeroton threshold per row (reference: 1e-9 + 1.5 noise_sd^2 / q^b) */
+    double sigma;               /* transform-domain noise sd per bin (noise_sd / sqrt(q^b)); 0: none */
+    double amax;                /* max |a_s| (> 0)                                              */
+    uint64_t seed;              /* Philox key of the request's noise                            */
+} qsft_request;
+
+typedef struct qsft_batch_sizes {
+    size_t input_bytes;         /* inputs arena: Mt, D, K, a of every request                   */
+    size_t work_bytes;          /* work arena: samples, working transform, energies, B fragments, peel workspace */
+    size_t plan_bytes;          /* descriptors and block prefixes (host plan and device plan)  */
+    size_t out_bytes;           /* packed results: header + cap_rows rows of row_stride bytes   */
+    int64_t cap_rows;
+    int32_t row_stride;
+    int32_t n_req;
+} qsft_batch_sizes;
+
+/* Device bytes one request holds while it runs (inputs + work): the admission unit. 0: bad shape. */
+size_t qsft_request_footprint(const qsft_request* r, int max_iter);
+/* NULL on a bad request (qsft_last_error says which). fuse_from: first peeling pass of the fused tail. */
+void* qsft_batch_create(const qsft_request* reqs, int n_req, int max_iter, int fuse_from, qsft_batch_sizes* sizes);
+/* Byte offsets of request i's Mt, D, K and a inside the inputs arena. */
+int qsft_batch_input_offset(void* batch, int i, size_t* offsets4);
+/* Request i's inputs into the host staging arena: M [C][n][b], D [R (n + 1)][n], locq [n][S] (int8 digits
+ * in [0, q), else QSFT_ERR_ARG) and a [S][2] (complex128). Also detects the identity / NSO delay structure
+ * (qsft_batch_delay_sign). */
+int qsft_batch_pack_input(void* batch, int i, const int8_t* M, const int8_t* D, const int8_t* locq,
+                          const double* a, void* input_host);
+int qsft_batch_delay_sign(void* batch, int i);
+int qsft_batch_bind(void* batch, void* input_dev, void* work_dev, void* plan_host, void* plan_dev, void* out);
+int qsft_batch_launch(void* batch, int upload, const void* input_host, uint64_t seed_offset, void* stream,
+                      void* tail_stream);
+/* 1: the last launch's rows are in `out`; 0: not yet; < 0: error. */
+int qsft_batch_done(void* batch);
+int qsft_batch_wait(void* batch);
+void qsft_batch_free(void* batch);
 
 /* ------------------------------------------------------------------------
  * No counterpart in the reference. For callers that cannot allocate device

@@ -78,7 +78,19 @@ class EvalMfmaDesc(ctypes.Structure):
                 ("Mt", vp), ("D", vp), ("K", vp), ("a", vp), ("Bf", vp), ("x", vp)]
 
 
-STRUCTS = {0: EvalDesc, 1: EvalPointsDesc, 2: DftDesc, 3: AssembleDesc, 4: PeelDesc, 5: EvalMfmaDesc}
+class Request(ctypes.Structure):
+    _fields_ = [("q", i32), ("n", i32), ("b", i32), ("C", i32), ("R", i32), ("S", i32), ("method", i32),
+                ("_pad0", i32), ("cutoff", ctypes.c_double), ("sigma", ctypes.c_double), ("amax", ctypes.c_double),
+                ("seed", ctypes.c_uint64)]
+
+
+class BatchSizes(ctypes.Structure):
+    _fields_ = [("input_bytes", ctypes.c_size_t), ("work_bytes", ctypes.c_size_t), ("plan_bytes", ctypes.c_size_t),
+                ("out_bytes", ctypes.c_size_t), ("cap_rows", i64), ("row_stride", i32), ("n_req", i32)]
+
+
+STRUCTS = {0: EvalDesc, 1: EvalPointsDesc, 2: DftDesc, 3: AssembleDesc, 4: PeelDesc, 5: EvalMfmaDesc, 6: Request,
+           7: BatchSizes}
 
 # exported symbols (every function declared in include/qsft_amd.h)
 SIGNATURES = {
@@ -118,6 +130,17 @@ SIGNATURES = {
     "qsft_peel_finalize": (ctypes.c_int, [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp]),
     "qsft_peel_pack_header_bytes": (ctypes.c_size_t, [ctypes.c_int]),
     "qsft_peel_pack": (ctypes.c_int, [vp, ctypes.c_int, vp, ctypes.c_int64, ctypes.c_int, vp]),
+    "qsft_peel_pack_ordered": (ctypes.c_int, [vp, ctypes.c_int, vp, ctypes.c_int64, ctypes.c_int, vp]),
+    "qsft_request_footprint": (ctypes.c_size_t, [ctypes.POINTER(Request), ctypes.c_int]),
+    "qsft_batch_create": (vp, [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(BatchSizes)]),
+    "qsft_batch_input_offset": (ctypes.c_int, [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_size_t)]),
+    "qsft_batch_pack_input": (ctypes.c_int, [vp, ctypes.c_int, vp, vp, vp, vp, vp]),
+    "qsft_batch_delay_sign": (ctypes.c_int, [vp, ctypes.c_int]),
+    "qsft_batch_bind": (ctypes.c_int, [vp, vp, vp, vp, vp, vp]),
+    "qsft_batch_launch": (ctypes.c_int, [vp, ctypes.c_int, vp, ctypes.c_uint64, vp, vp]),
+    "qsft_batch_done": (ctypes.c_int, [vp]),
+    "qsft_batch_wait": (ctypes.c_int, [vp]),
+    "qsft_batch_free": (None, [vp]),
     "qsft_noalloc_scratch_bytes": (ctypes.c_size_t, []),
     "qsft_peel_transform_noalloc": (ctypes.c_int, [ctypes.c_int] * 8 + [ctypes.c_double, ctypes.c_int, vp, vp, vp,
                                                                         ctypes.c_int, vp, ctypes.c_int64,

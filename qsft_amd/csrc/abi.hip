@@ -32,6 +32,8 @@ extern "C" size_t qsft_sizeof(int which) {
         case 3: return sizeof(qsft_assemble_desc);
         case 4: return sizeof(qsft_peel_desc);
         case 5: return sizeof(qsft_eval_mfma_desc);
+        case 6: return sizeof(qsft_request);
+        case 7: return sizeof(qsft_batch_sizes);
         default: return 0;
     }
 }

@@ -1318,6 +1318,47 @@ __global__ __launch_bounds__(256) void k_pack_rows(const qsft_peel_desc* __restr
     }
 }
 
+// The same rows, each written at its instance offset plus its rank in first
+// detection order (the reference's dict insertion order), so a caller reads
+// every instance's rows already ordered. The rank of a row is the number of
+// the instance's rows with a smaller order word (order words are distinct:
+// one first detection per pass and bin), counted against LDS tiles of the
+// order words. One block per instance.
+constexpr int kRankTile = 2048;
+
+__global__ __launch_bounds__(256) void k_pack_rows_ordered(const qsft_peel_desc* __restrict__ descs, int n_inst,
+                                                           const int64_t* __restrict__ hdr,
+                                                           unsigned char* __restrict__ rows, int64_t cap_rows,
+                                                           int row_stride) {
+    __shared__ uint64_t tile[kRankTile];
+    const int i = blockIdx.x;
+    const qsft_peel_desc& d = descs[i];
+    const int64_t m = hdr[i], off = hdr[n_inst + i];
+    const int nw = d.n_pad >> 2;
+    for (int64_t r0 = 0; r0 < m; r0 += blockDim.x) {              // block-uniform loop
+        const int64_t r = r0 + threadIdx.x;
+        const uint64_t mine = r < m ? d.out_order[r] : ~0ull;
+        int64_t rank = 0;
+        for (int64_t t0 = 0; t0 < m; t0 += kRankTile) {
+            const int tn = (int)min((int64_t)kRankTile, m - t0);
+            __syncthreads();                                       // the previous tile is consumed
+            for (int j = threadIdx.x; j < tn; j += blockDim.x) tile[j] = d.out_order[t0 + j];
+            __syncthreads();
+            for (int j = 0; j < tn; ++j) rank += tile[j] < mine ? 1 : 0;
+        }
+        if (r < m && off + rank < cap_rows) {
+            unsigned char* dst = rows + (off + rank) * row_stride;
+            const uint32_t* ks = reinterpret_cast<const uint32_t*>(d.out_k + r * d.n_pad);
+            uint32_t* kd = reinterpret_cast<uint32_t*>(dst);
+            for (int w = 0; w < nw; ++w) kd[w] = ks[w];
+            double* vd = reinterpret_cast<double*>(dst + d.n_pad);
+            vd[0] = d.out_val[2 * r];
+            vd[1] = d.out_val[2 * r + 1];
+            *reinterpret_cast<uint64_t*>(dst + d.n_pad + 16) = mine;
+        }
+    }
+}
+
 }  // namespace qsft
 
 using namespace qsft;
@@ -1540,4 +1581,21 @@ extern "C" int qsft_peel_pack(const qsft_peel_desc* d, int n_inst, void* dst, in
     if (e != hipSuccess) return qsft_hip_status(e, "qsft_peel_pack(scan)");
     hipLaunchKernelGGL(k_pack_rows, dim3(n_inst), dim3(256), 0, st, d, n_inst, hdr, rows, cap_rows, row_stride);
     return qsft_hip_status(hipGetLastError(), "qsft_peel_pack(rows)");
+}
+
+extern "C" int qsft_peel_pack_ordered(const qsft_peel_desc* d, int n_inst, void* dst, int64_t cap_rows,
+                                      int row_stride, void* stream) {
+    if (!d || !dst || n_inst < 1 || cap_rows < 0 || row_stride < 24 + 64 || row_stride % 8) {
+        qsft_set_error("qsft_peel_pack_ordered: bad arguments");
+        return QSFT_ERR_ARG;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    int64_t* hdr = static_cast<int64_t*>(dst);
+    unsigned char* rows = static_cast<unsigned char*>(dst) + qsft_peel_pack_header_bytes(n_inst);
+    hipLaunchKernelGGL(k_pack_scan, dim3(1), dim3(64), 0, st, d, n_inst, hdr, cap_rows);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return qsft_hip_status(e, "qsft_peel_pack_ordered(scan)");
+    hipLaunchKernelGGL(k_pack_rows_ordered, dim3(n_inst), dim3(256), 0, st, d, n_inst, hdr, rows, cap_rows,
+                       row_stride);
+    return qsft_hip_status(hipGetLastError(), "qsft_peel_pack_ordered(rows)");
 }

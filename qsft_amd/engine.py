@@ -33,7 +33,7 @@ class TransformSpec:
     b: int
     C: int
     R: int
-    locq: np.ndarray                  # (n, S) support digits
+    locq: np.ndarray                  # (n, S) support digits (int8 from random_spec)
     strengths: np.ndarray             # (S,) complex
     Ms: list                          # C arrays (n, b)
     D: np.ndarray                     # (R * (n + 1), n) delay rows shared by all groups
@@ -60,6 +60,17 @@ class TransformSpec:
             return self.cutoff
         return 1e-9 + 1.5 * self.noise_sd ** 2 / self.q ** self.b
 
+    def wire(self):
+        """The request's payload as the serving runtime reads it, built once:
+        M int8 [C][n][b], D int8 [P][n], locq int8 [n][S], a complex128 [S]."""
+        w = getattr(self, "_wire", None)
+        if w is None:
+            M = np.ascontiguousarray(np.stack([np.asarray(m)[:, :self.b] for m in self.Ms]), dtype=np.int8)
+            w = (M, np.ascontiguousarray(self.D, dtype=np.int8), np.ascontiguousarray(self.locq, dtype=np.int8),
+                 np.ascontiguousarray(self.strengths, dtype=np.complex128))
+            self._wire = w
+        return w
+
     def footprint_bytes(self, precision="fp32"):
         """HBM bytes this transform holds while resident (samples/transform, FP64
         working copy, peel workspace, inputs)."""
@@ -75,7 +86,7 @@ def random_spec(q, n, S, b, C=3, R=2, snr_db=10.0, rng=None, seed=0, distinct=Tr
     |a| ~ U[1, 10], phase ~ U[0, 2 pi), uniform M_c, NSO delays with R random
     offsets, noise_sd^2 = sum |a|^2 / 10^(snr/10) (None -> noiseless)."""
     rng = np.random.default_rng(seed) if rng is None else rng
-    locq = rng.integers(0, q, size=(n, S))
+    locq = rng.integers(0, q, size=(n, S)).astype(np.int8)        # int8 digits, the same stream as int64 draws
     if distinct:
         while True:
             keys = {tuple(c) for c in locq.T}

@@ -1,0 +1,307 @@
+"""Serving loop: transforms of mixed shapes admitted as they arrive, batched
+continuously onto the GPU, results delivered per request.
+
+The reference serves one request per `QSFT.transform` call
+(`qsft/qsft.py:56-281`) and scales only through process pools
+(`qsft/parallel_tests.py:91-97`). Here a request is a `TransformSpec`
+(q, n, b, S, C, R, noise), and a `Server`:
+
+  * admits it by its HBM footprint (`qsft_request_footprint`: inputs,
+    samples, FP64 working transform, B fragments, peel workspace) against a
+    byte budget; a request that could never fit is rejected at submit
+    (`AdmissionError`), one that does not fit now waits in arrival order;
+  * keeps up to `max_inflight` batches on the GPU at once, each on its own
+    stream with its finalize/pack on a tail stream, so a batch formed from
+    requests that arrived while others run overlaps their work on the device;
+    a batch that completes frees its slot and bytes at once and the next
+    batch is formed from everything that fits (continuous batching);
+  * builds each batch with the native batch runtime (`qsft_batch_*`: one
+    staging arena, one plan, one upload, every stage enqueued by one call);
+  * delivers each request's recovered (k, value) rows as views of the packed
+    output in pinned host memory, already in the reference's dict order
+    (first detection); `Result.to_dict()` builds the reference's dict on
+    demand.
+
+Latency is measured per request from its arrival (submit time) to the moment
+its batch's results are in host memory and handed to it.
+"""
+import collections
+import ctypes
+import time
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import _lib as L
+from .engine import TransformSpec
+
+lib = L.lib
+
+
+class AdmissionError(ValueError):
+    """The request's HBM footprint exceeds the server's whole budget."""
+
+
+def request_struct(sp, max_iter=15):
+    """The qsft_request of a TransformSpec."""
+    if sp.method == "nso":
+        method = L.QSFT_DETECT_NSO1
+    elif sp.method == "identity":
+        method = L.QSFT_DETECT_NOISELESS
+    else:
+        raise ValueError(f"detection method {sp.method!r}")
+    Q = sp.q ** sp.b
+    a = sp.strengths
+    amax = float(np.sqrt(np.max(a.real * a.real + a.imag * a.imag))) if a.size else 0.0
+    return L.Request(sp.q, sp.n, sp.b, sp.C, sp.R, a.size, method, 0, float(sp.threshold()),
+                     float(sp.noise_sd) / Q ** 0.5 if sp.noise_sd > 0 else 0.0, amax,
+                     int(sp.seed) & 0xFFFFFFFFFFFFFFFF)
+
+
+def footprint(sp, max_iter=15):
+    """Device bytes the request holds while its batch runs (0: not servable)."""
+    r = request_struct(sp, max_iter)
+    return int(lib.qsft_request_footprint(ctypes.byref(r), max_iter))
+
+
+class Result:
+    """One request's recovered spectrum: rows in first-detection order.
+
+    `k` (int8 [m][n]) and `values` (complex128 [m]) are views of the batch's
+    packed output in pinned host memory (kept alive by this object)."""
+
+    __slots__ = ("k", "values", "order", "iterations")
+
+    def __init__(self, k, values, order, iterations):
+        self.k, self.values, self.order, self.iterations = k, values, order, iterations
+
+    def to_dict(self):
+        """The reference's {tuple(k): coefficient} dict (qsft/qsft.py:247-272)."""
+        return {tuple(int(x) for x in row): complex(v) for row, v in zip(self.k, self.values)}
+
+
+class ServedBatch:
+    """One batch of requests on the native runtime: staged, bound, launched."""
+
+    def __init__(self, specs, device, stream=None, tail_stream=None, max_iter=15, fuse_from=3, structs=None):
+        self.specs = list(specs)
+        n = len(self.specs)
+        if structs is None:
+            structs = [request_struct(sp, max_iter) for sp in self.specs]
+        self.reqs = (L.Request * n)(*structs)
+        sz = L.BatchSizes()
+        h = lib.qsft_batch_create(ctypes.cast(self.reqs, ctypes.c_void_p), n, max_iter, fuse_from, ctypes.byref(sz))
+        if not h:
+            raise L.QsftLibraryError("qsft_batch_create: " + lib.qsft_last_error().decode(errors="replace"))
+        self.h = ctypes.c_void_p(h)
+        self.sizes = sz
+        self.bytes = int(sz.input_bytes + sz.work_bytes + sz.plan_bytes)
+        pin = dict(dtype=torch.uint8, pin_memory=True)
+        self.in_host = torch.empty(max(int(sz.input_bytes), 256), **pin)
+        self.plan_host = torch.empty(max(int(sz.plan_bytes), 256), **pin)
+        self.out = torch.empty(int(sz.out_bytes), **pin)
+        self.in_dev = torch.empty(max(int(sz.input_bytes), 256), dtype=torch.uint8, device=device)
+        self.work_dev = torch.empty(max(int(sz.work_bytes), 256), dtype=torch.uint8, device=device)
+        self.plan_dev = torch.empty(max(int(sz.plan_bytes), 256), dtype=torch.uint8, device=device)
+        base = self.in_host.data_ptr()
+        pack = lib.qsft_batch_pack_input
+        for i, sp in enumerate(self.specs):
+            M, D, locq, a = sp.wire()
+            rc = pack(self.h, i, M.ctypes.data, D.ctypes.data, locq.ctypes.data, a.ctypes.data, base)
+            if rc:
+                L.check(rc, "qsft_batch_pack_input")
+        L.check(lib.qsft_batch_bind(self.h, self.in_dev.data_ptr(), self.work_dev.data_ptr(),
+                                    self.plan_host.data_ptr(), self.plan_dev.data_ptr(), self.out.data_ptr()),
+                "qsft_batch_bind")
+        self.stream = stream
+        self.tail_stream = tail_stream
+        self.uploaded = False
+
+    def launch(self, seed_offset=0):
+        """Enqueue the whole pipeline (the first launch also uploads inputs and plan)."""
+        st = (self.stream or torch.cuda.current_stream()).cuda_stream
+        ts = self.tail_stream.cuda_stream if self.tail_stream is not None else None
+        upload = 0 if self.uploaded else 3
+        L.check(lib.qsft_batch_launch(self.h, upload, self.in_host.data_ptr(), int(seed_offset) & (2 ** 64 - 1),
+                                      st, ts), "qsft_batch_launch")
+        self.uploaded = True
+
+    def done(self):
+        rc = lib.qsft_batch_done(self.h)
+        if rc < 0:
+            raise L.QsftLibraryError("qsft_batch_done: " + lib.qsft_last_error().decode(errors="replace"))
+        return rc == 1
+
+    def wait(self):
+        L.check(lib.qsft_batch_wait(self.h), "qsft_batch_wait")
+
+    def results(self):
+        """Per request a Result (views of the pinned output) once the batch is done."""
+        n = len(self.specs)
+        hb = int(lib.qsft_peel_pack_header_bytes(n))
+        buf = self.out.numpy()
+        hdr = buf[:hb].view(np.int64)
+        err = int(hdr[2 * n + 1])
+        if err:
+            raise L.QsftLibraryError(f"peel device error bits {err:#x} (1 table full, 2 tag collision, "
+                                     "4 output full)")
+        stride = int(self.sizes.row_stride)
+        total = int(hdr[2 * n])
+        rows = buf[hb:hb + total * stride]
+        iters = int(hdr[2 * n + 2])
+        out = []
+        for i, sp in enumerate(self.specs):
+            cnt, off = int(hdr[i]), int(hdr[n + i])
+            n_pad = (sp.n + 63) // 64 * 64
+            blk = rows[off * stride:(off + cnt) * stride]
+            k = np.ndarray((cnt, sp.n), dtype=np.int8, buffer=blk, strides=(stride, 1)) if cnt else \
+                np.zeros((0, sp.n), np.int8)
+            vals = np.ndarray((cnt,), dtype=np.complex128, buffer=blk, offset=n_pad, strides=(stride,)) if cnt \
+                else np.zeros(0, np.complex128)
+            order = np.ndarray((cnt,), dtype=np.uint64, buffer=blk, offset=n_pad + 16, strides=(stride,)) if cnt \
+                else np.zeros(0, np.uint64)
+            out.append(Result(k, vals, order, iters))
+        return out
+
+    def free(self):
+        if self.h:
+            lib.qsft_batch_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+@dataclass
+class Request:
+    spec: TransformSpec
+    footprint: int
+    t_submit: float
+    struct: object = None
+    t_start: float = None
+    t_done: float = None
+    result: Result = None
+    batch_size: int = 0
+
+    @property
+    def latency(self):
+        return None if self.t_done is None else self.t_done - self.t_submit
+
+
+class Server:
+    """Continuous-batching server on one GPU.
+
+    hbm_budget: bytes that requests in flight may hold together (admission);
+    max_batch: requests per batch; max_inflight: batches on the GPU at once
+    (each on its own stream; finalize and pack on a tail stream per slot)."""
+
+    def __init__(self, hbm_budget, max_batch=64, max_inflight=2, max_iter=15, fuse_from=3, device=None):
+        self.device = device if device is not None else torch.device("cuda", torch.cuda.current_device())
+        self.budget = int(hbm_budget)
+        self.max_batch = int(max_batch)
+        self.max_inflight = int(max_inflight)
+        self.max_iter, self.fuse_from = max_iter, fuse_from
+        self.waiting = collections.deque()
+        self.inflight = []                    # (ServedBatch, [Request], slot)
+        self.used = 0
+        self.streams = [torch.cuda.Stream(device=self.device) for _ in range(self.max_inflight)]
+        self.tails = [torch.cuda.Stream(device=self.device) for _ in range(self.max_inflight)]
+        self.free_slots = list(range(self.max_inflight))
+        self.completed = []
+        self.batches = 0
+
+    def submit(self, spec, t_submit=None):
+        rs = request_struct(spec, self.max_iter)
+        fp = int(lib.qsft_request_footprint(ctypes.byref(rs), self.max_iter))
+        if fp <= 0:
+            raise AdmissionError(f"request shape not servable (q={spec.q}, n={spec.n}, b={spec.b}, R={spec.R})")
+        if fp > self.budget:
+            raise AdmissionError(f"request needs {fp} bytes of HBM, more than the whole budget {self.budget}")
+        req = Request(spec=spec, footprint=fp, t_submit=time.perf_counter() if t_submit is None else t_submit,
+                      struct=rs)
+        self.waiting.append(req)
+        return req
+
+    def _retire(self):
+        still = []
+        for bt, reqs, slot in self.inflight:
+            if bt.done():
+                res = bt.results()
+                now = time.perf_counter()
+                for r, x in zip(reqs, res):
+                    r.result = x
+                    r.t_done = now
+                self.completed.extend(reqs)
+                self.used -= bt.bytes
+                self.free_slots.append(slot)
+                bt.free()
+            else:
+                still.append((bt, reqs, slot))
+        self.inflight = still
+
+    def _admit(self):
+        while self.waiting and self.free_slots:
+            take, used = [], self.used
+            while self.waiting and len(take) < self.max_batch:
+                r = self.waiting[0]
+                if used + r.footprint > self.budget:
+                    break                                          # arrival order: wait for bytes to free up
+                take.append(self.waiting.popleft())
+                used += r.footprint
+            if not take:
+                return
+            slot = self.free_slots.pop()
+            bt = ServedBatch([r.spec for r in take], self.device, stream=self.streams[slot],
+                             tail_stream=self.tails[slot], max_iter=self.max_iter, fuse_from=self.fuse_from,
+                             structs=[r.struct for r in take])
+            now = time.perf_counter()
+            for r in take:
+                r.t_start = now
+                r.batch_size = len(take)
+            bt.launch()
+            self.used += bt.bytes
+            self.inflight.append((bt, take, slot))
+            self.batches += 1
+
+    def poll(self):
+        """Retire finished batches, then start new ones from the waiting requests."""
+        self._retire()
+        self._admit()
+
+    def idle(self):
+        return not self.waiting and not self.inflight
+
+    def drain(self):
+        while not self.idle():
+            self.poll()
+            if self.inflight and not self.free_slots:
+                time.sleep(20e-6)
+
+    def run_open_loop(self, arrivals, specs):
+        """Replay arrivals (seconds from now, ascending) of `specs`: submit each
+        at its time, keep polling, until every request has its result.
+        Returns the Requests in arrival order."""
+        t0 = time.perf_counter()
+        reqs = []
+        i, n = 0, len(specs)
+        while i < n or not self.idle():
+            now = time.perf_counter()
+            while i < n and t0 + arrivals[i] <= now:
+                reqs.append(self.submit(specs[i], t_submit=t0 + arrivals[i]))
+                i += 1
+            self.poll()
+            if i < n and not self.inflight and not self.waiting:
+                wait = t0 + arrivals[i] - time.perf_counter()
+                if wait > 50e-6:
+                    time.sleep(wait - 30e-6)
+        return reqs
+
+
+def latency_summary(reqs):
+    lat = np.array([r.latency for r in reqs], dtype=np.float64)
+    return {"p50_ms": 1e3 * float(np.percentile(lat, 50)), "p99_ms": 1e3 * float(np.percentile(lat, 99)),
+            "mean_ms": 1e3 * float(lat.mean()), "n": int(lat.size)}

@@ -84,3 +84,34 @@ def test_peel_workspace_and_bind():
                                 None, None, None, base, nbytes, 64) == L.QSFT_ERR_ARG
     assert L.lib.qsft_peel_bind(ctypes.byref(d), 4, 50, 6, 3, 102, 51, L.QSFT_DETECT_NSO1, 15, 1.0, 0,
                                 None, None, None, base, 1000, 64) == L.QSFT_ERR_ARG
+
+
+def test_batch_runtime_sizes_on_the_host():
+    """The batch runtime plans on the host: a batch's two arenas hold exactly
+    the per-request footprints (the admission unit), and a bad request is
+    refused with a message."""
+    reqs = [L.Request(q=4, n=50, b=6, C=3, R=2, S=1000, method=L.QSFT_DETECT_NSO1, cutoff=1.3, sigma=0.9,
+                      amax=10.0, seed=5),
+            L.Request(q=20, n=16, b=3, C=3, R=2, S=2000, method=L.QSFT_DETECT_NSO1, cutoff=1.0, sigma=1.0,
+                      amax=9.0, seed=6),
+            L.Request(q=3, n=12, b=4, C=3, R=1, S=40, method=L.QSFT_DETECT_NOISELESS, cutoff=1e-9, sigma=0.0,
+                      amax=3.0, seed=7)]
+    fps = [L.lib.qsft_request_footprint(ctypes.byref(r), 15) for r in reqs]
+    assert 40 << 20 < fps[0] < 48 << 20                          # K1: 24 B per sample + peel workspace
+    arr = (L.Request * 3)(*reqs)
+    sz = L.BatchSizes()
+    h = L.lib.qsft_batch_create(ctypes.cast(arr, ctypes.c_void_p), 3, 15, 3, ctypes.byref(sz))
+    assert h
+    try:
+        assert sz.input_bytes + sz.work_bytes == sum(fps)
+        assert sz.n_req == 3 and sz.row_stride == 64 + 24 and sz.cap_rows == sum(4 * r.S + 1024 for r in reqs)
+        assert 0 < sz.plan_bytes < 1 << 16
+        offs = (ctypes.c_size_t * 4)()
+        assert L.lib.qsft_batch_input_offset(h, 1, offs) == L.QSFT_OK
+        assert list(offs) == sorted(offs) and all(o % 256 == 0 for o in offs)
+    finally:
+        L.lib.qsft_batch_free(h)
+    bad = (L.Request * 1)(L.Request(q=4, n=10, b=4, C=3, R=2, S=10, method=L.QSFT_DETECT_NOISELESS, cutoff=1.0,
+                                    sigma=0.0, amax=1.0, seed=0))              # noiseless detection needs R = 1
+    assert not L.lib.qsft_batch_create(ctypes.cast(bad, ctypes.c_void_p), 1, 15, 3, ctypes.byref(sz))
+    assert b"bad request shape" in L.lib.qsft_last_error()

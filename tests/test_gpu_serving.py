@@ -1,0 +1,155 @@
+"""The serving loop on the GPU: the native batch runtime gives the engine's
+rows; every request's result is the same alone and in the mix; admission by
+HBM footprint rejects what can never fit and queues what does not fit yet."""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+if torch.cuda.is_available():
+    from qsft_amd import engine as E
+    from qsft_amd import serving as SV
+
+
+def _rows(k, v):
+    return [tuple(int(x) for x in r) for r in k], np.ascontiguousarray(v, dtype=np.complex128)
+
+
+def _bits(v):
+    return np.ascontiguousarray(v, dtype=np.complex128).view(np.float64)
+
+
+def _k3_mix(count, rng, large=0.0):
+    """K3 recipe: q in {2, 3, 4, 20}, n in [8, 64], S log-uniform in [10, 2000]
+    (at most q^n / 8), b = ceil(log_q S) + 1, C = 3, NSO R = 2, SNR U[5, 30] dB;
+    a `large` share of q = 4, n = 100, S = 5000, b = 7 jobs."""
+    specs = []
+    for _ in range(count):
+        if rng.uniform() < large:
+            specs.append(E.random_spec(4, 100, 5000, 7, 3, 2, float(rng.uniform(5, 30)), rng=rng))
+            continue
+        q = int(rng.choice([2, 3, 4, 20]))
+        n = int(rng.integers(8, 65))
+        S = int(np.exp(rng.uniform(np.log(10), np.log(2000))))
+        S = max(2, min(S, q ** n // 8))
+        b = int(np.ceil(np.log(S) / np.log(q))) + 1
+        specs.append(E.random_spec(q, n, S, b, 3, 2, float(rng.uniform(5, 30)), rng=rng))
+    return specs
+
+
+def test_runtime_batch_equals_engine_batch():
+    """One native-runtime batch (ordered pack) against the Python engine's
+    batch on the same inputs and noise: the same rows, bits and order."""
+    rng = np.random.default_rng(3)
+    specs = [E.random_spec(4, 50, 1000, 6, 3, 2, snr, rng=rng) for snr in (3.0, 10.0, None)]
+    specs += [E.random_spec(20, 16, 300, 3, 3, 2, 20.0, rng=rng), E.random_spec(3, 20, 90, 4, 3, 2, 15.0, rng=rng),
+              E.random_spec(2, 70, 60, 7, 3, 2, 12.0, rng=rng),                      # P = 142
+              E.random_spec(4, 12, 100, 4, 3, 1, None, rng=rng, method="identity")]
+    print("runtime vs engine", flush=True)
+    dev = torch.device("cuda", torch.cuda.current_device())
+    sb = SV.ServedBatch(specs, dev)
+    sb.launch(seed_offset=6)
+    sb.wait()
+    got = sb.results()
+    ref = E.Batch(specs, precision="fp32")
+    ref.launch(seed_offset=6)
+    want = ref.results()
+    for i, (g, (k, v, order, _)) in enumerate(zip(got, want)):
+        kg, vg = _rows(g.k, g.values)
+        kw, vw = _rows(k, v)
+        assert kg == kw, i
+        assert np.array_equal(_bits(vg), _bits(vw)), i
+        assert np.array_equal(np.asarray(g.order), order), i
+    # a relaunch with another seed offset draws fresh noise; the same offset repeats the bits
+    sb.launch(seed_offset=7)
+    sb.wait()
+    again = sb.results()
+    assert any(not np.array_equal(a.values, b.values) for a, b in zip(again[:2], got[:2]))
+    sb.launch(seed_offset=6)
+    sb.wait()
+    for a, b in zip(sb.results(), got):
+        assert np.array_equal(_bits(a.values), _bits(b.values))
+    # the noiseless requests recover their planted spectrum
+    for i in (2, 6):
+        d = got[i].to_dict()
+        truth = {tuple(int(x) for x in c): a for c, a in zip(specs[i].locq.T, specs[i].strengths)}
+        assert set(d) == set(truth)
+        assert max(abs(d[k] - truth[k]) / abs(truth[k]) for k in truth) < 1e-5
+
+
+def test_each_result_is_the_same_alone_and_in_the_mix():
+    """40 requests of the K3 mix plus one large job arrive together and are
+    served in continuous batches (at most 16 per batch, two batches on the GPU
+    at once). Each request's rows equal those of the request served alone
+    (same noise seed): same keys in the same order, values within 1e-12."""
+    rng = np.random.default_rng(44)
+    specs = _k3_mix(40, rng) + [E.random_spec(4, 100, 5000, 7, 3, 2, 10.0, rng=rng)]
+    print("served mix of", len(specs), flush=True)
+    srv = SV.Server(hbm_budget=40 << 30, max_batch=16, max_inflight=2)
+    reqs = [srv.submit(sp) for sp in specs]
+    srv.drain()
+    assert srv.batches >= 3
+    dev = torch.device("cuda", torch.cuda.current_device())
+    for i, (sp, r) in enumerate(zip(specs, reqs)):
+        alone = SV.ServedBatch([sp], dev)
+        alone.launch()
+        alone.wait()
+        a, = alone.results()
+        kg, vg = _rows(r.result.k, r.result.values)
+        ka, va = _rows(a.k, a.values)
+        assert kg == ka, (i, sp.q, sp.n, sp.b, sp.S)
+        assert np.all(np.abs(vg - va) <= 1e-12 * np.abs(va)), i
+        alone.free()
+    lat = SV.latency_summary(reqs)
+    print(f"{srv.batches} batches, latency p50 {lat['p50_ms']:.2f} ms p99 {lat['p99_ms']:.2f} ms", flush=True)
+
+
+def test_admission_rejects_oversize_and_queues_under_a_small_budget():
+    rng = np.random.default_rng(45)
+    small = [E.random_spec(4, 30, 200, 5, 3, 2, 15.0, rng=rng) for _ in range(9)]
+    fp = SV.footprint(small[0])
+    budget = int(2.5 * fp)                                   # two such requests at a time
+    srv = SV.Server(hbm_budget=budget, max_batch=8, max_inflight=2)
+    big = E.random_spec(4, 50, 1000, 6, 3, 2, 10.0, rng=rng)
+    assert SV.footprint(big) > budget
+    with pytest.raises(SV.AdmissionError):
+        srv.submit(big)
+    reqs = [srv.submit(sp) for sp in small]
+    peak = 0
+    while not srv.idle():
+        srv.poll()
+        peak = max(peak, srv.used)
+        assert srv.used <= budget
+    assert all(r.result is not None for r in reqs)
+    assert srv.batches >= 5                                    # at most two requests per batch fit
+    print(f"budget {budget} bytes, peak {peak}, {srv.batches} batches", flush=True)
+    dev = torch.device("cuda", torch.cuda.current_device())
+    for sp, r in zip(small, reqs):
+        alone = SV.ServedBatch([sp], dev)
+        alone.launch()
+        alone.wait()
+        a, = alone.results()
+        assert _rows(r.result.k, r.result.values)[0] == _rows(a.k, a.values)[0]
+
+
+def test_open_loop_poisson_arrivals_complete():
+    """Poisson arrivals replayed against the clock: every request completes,
+    latencies are measured from arrival, noiseless requests are exact."""
+    rng = np.random.default_rng(46)
+    specs = _k3_mix(60, rng)
+    for sp in specs[:10]:
+        sp.noise_sd = 0.0
+    arrivals = np.cumsum(rng.exponential(1.0 / 2000.0, size=len(specs)))
+    srv = SV.Server(hbm_budget=40 << 30, max_batch=32, max_inflight=2)
+    reqs = srv.run_open_loop(arrivals, specs)
+    assert len(reqs) == len(specs) and all(r.result is not None for r in reqs)
+    assert all(r.t_done >= r.t_submit for r in reqs)
+    for sp, r in zip(specs[:10], reqs[:10]):
+        d = r.result.to_dict()
+        truth = {tuple(int(x) for x in c): a for c, a in zip(sp.locq.T, sp.strengths)}
+        assert set(d) <= set(truth)
+        for k, v in d.items():
+            assert abs(v - truth[k]) <= 1e-4 * abs(truth[k])
+    lat = SV.latency_summary(reqs)
+    print(f"open loop: {srv.batches} batches, p50 {lat['p50_ms']:.2f} ms p99 {lat['p99_ms']:.2f} ms", flush=True)
