@@ -157,11 +157,30 @@ typedef struct qsft_dft_desc {
     int32_t n_pass;
     int32_t pass_d0[QSFT_DFT_MAX_PASSES];   /* first axis of the pass                       */
     int32_t pass_d1[QSFT_DFT_MAX_PASSES];   /* one past the last axis                       */
-    int32_t pass_ti[QSFT_DFT_MAX_PASSES];   /* contiguous inner elements per tile           */
+    int32_t pass_ti[QSFT_DFT_MAX_PASSES];   /* contiguous inner elements per tile (divides I) */
     int32_t pass_blocks[QSFT_DFT_MAX_PASSES];
+    int32_t pass_ta[QSFT_DFT_MAX_PASSES];   /* whole [M, I] blocks per tile (pass_ti = I only) */
 } qsft_dft_desc;
 
+/* Fills the pass fields: the schedule qsft_dft_autotune measured fastest for
+ * (q, b, precision) in this process, else the planner's model-ranked best
+ * (every split of the b axes into at most 4 passes, under several LDS budgets
+ * and inner-run lengths). Every schedule applies the same axis transforms in
+ * the same order (innermost axis first), so results do not depend on it. */
 int qsft_dft_plan(qsft_dft_desc* host_desc, int precision);
+/* The planner's candidate `index` (0 = the model's best, in the order the
+ * autotuner tries them); QSFT_ERR_ARG past the last. */
+int qsft_dft_plan_candidate(qsft_dft_desc* host_desc, int precision, int index);
+/* Autotuner: times the model's best-ranked schedules (at most max_candidates)
+ * on `rows` rows held in `buf` (rows * q^b elements + 4096 bytes, device
+ * memory, overwritten) on `stream`, synchronously, and records the fastest
+ * for qsft_dft_plan; *best_ms = its time for the rows. Call once per shape
+ * before the launches that should use it (not under graph capture). */
+int qsft_dft_autotune(int q, int b, int precision, int rows, void* buf, size_t buf_bytes, int max_candidates,
+                      void* stream, float* best_ms);
+/* The schedule qsft_dft_plan returns for (q, b, precision): pass count, axis
+ * groups [d0, d1), TI and TA per pass (arrays of 4); *tuned = 1 if measured. */
+int qsft_dft_schedule(int q, int b, int precision, int* n_pass, int* d0, int* d1, int* ti, int* ta, int* tuned);
 /* Dynamic LDS bytes one instance's pass needs (tile + twiddles, <= 64 KiB). */
 int qsft_dft_lds_bytes(const qsft_dft_desc* host_desc, int pass, int precision);
 /* lds_bytes: dynamic LDS of the launch, at least qsft_dft_lds_bytes() of every

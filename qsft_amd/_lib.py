@@ -46,7 +46,8 @@ class DftDesc(ctypes.Structure):
     _fields_ = [("q", i32), ("b", i32), ("rows", i32), ("in_stride", i32), ("in_row_stride", i64),
                 ("in_", vp), ("out", vp), ("n_pass", i32),
                 ("pass_d0", i32 * QSFT_DFT_MAX_PASSES), ("pass_d1", i32 * QSFT_DFT_MAX_PASSES),
-                ("pass_ti", i32 * QSFT_DFT_MAX_PASSES), ("pass_blocks", i32 * QSFT_DFT_MAX_PASSES)]
+                ("pass_ti", i32 * QSFT_DFT_MAX_PASSES), ("pass_blocks", i32 * QSFT_DFT_MAX_PASSES),
+                ("pass_ta", i32 * QSFT_DFT_MAX_PASSES)]
 
 
 class AssembleDesc(ctypes.Structure):
@@ -109,6 +110,10 @@ SIGNATURES = {
     "qsft_eval_points": (ctypes.c_int, [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp]),
     "qsft_dft_plan": (ctypes.c_int, [ctypes.POINTER(DftDesc), ctypes.c_int]),
     "qsft_dft_lds_bytes": (ctypes.c_int, [ctypes.POINTER(DftDesc), ctypes.c_int, ctypes.c_int]),
+    "qsft_dft_plan_candidate": (ctypes.c_int, [ctypes.POINTER(DftDesc), ctypes.c_int, ctypes.c_int]),
+    "qsft_dft_autotune": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, ctypes.c_size_t,
+                                         ctypes.c_int, vp, ctypes.POINTER(ctypes.c_float)]),
+    "qsft_dft_schedule": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int] + [ctypes.POINTER(ctypes.c_int)] * 6),
     "qsft_dft_pass": (ctypes.c_int, [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                      vp]),
     "qsft_dft4_rows": (ctypes.c_int, [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp]),

@@ -1323,19 +1323,22 @@ __global__ __launch_bounds__(256) void k_pack_rows(const qsft_peel_desc* __restr
 // every instance's rows already ordered. The rank of a row is the number of
 // the instance's rows with a smaller order word (order words are distinct:
 // one first detection per pass and bin), counted against LDS tiles of the
-// order words. One block per instance.
+// order words. kRankGroups blocks share an instance (block g ranks rows
+// g*256 + tid, stepping by kRankGroups*256), so an instance with thousands
+// of rows spreads its m^2 comparisons over many CUs.
 constexpr int kRankTile = 2048;
+constexpr int kRankGroups = 32;
 
 __global__ __launch_bounds__(256) void k_pack_rows_ordered(const qsft_peel_desc* __restrict__ descs, int n_inst,
                                                            const int64_t* __restrict__ hdr,
                                                            unsigned char* __restrict__ rows, int64_t cap_rows,
                                                            int row_stride) {
     __shared__ uint64_t tile[kRankTile];
-    const int i = blockIdx.x;
+    const int i = blockIdx.x / kRankGroups, g = blockIdx.x % kRankGroups;
     const qsft_peel_desc& d = descs[i];
     const int64_t m = hdr[i], off = hdr[n_inst + i];
     const int nw = d.n_pad >> 2;
-    for (int64_t r0 = 0; r0 < m; r0 += blockDim.x) {              // block-uniform loop
+    for (int64_t r0 = (int64_t)g * blockDim.x; r0 < m; r0 += (int64_t)kRankGroups * blockDim.x) {   // block-uniform
         const int64_t r = r0 + threadIdx.x;
         const uint64_t mine = r < m ? d.out_order[r] : ~0ull;
         int64_t rank = 0;
@@ -1595,7 +1598,7 @@ extern "C" int qsft_peel_pack_ordered(const qsft_peel_desc* d, int n_inst, void*
     hipLaunchKernelGGL(k_pack_scan, dim3(1), dim3(64), 0, st, d, n_inst, hdr, cap_rows);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return qsft_hip_status(e, "qsft_peel_pack_ordered(scan)");
-    hipLaunchKernelGGL(k_pack_rows_ordered, dim3(n_inst), dim3(256), 0, st, d, n_inst, hdr, rows, cap_rows,
-                       row_stride);
+    hipLaunchKernelGGL(k_pack_rows_ordered, dim3(n_inst * kRankGroups), dim3(256), 0, st, d, n_inst, hdr, rows,
+                       cap_rows, row_stride);
     return qsft_hip_status(hipGetLastError(), "qsft_peel_pack_ordered(rows)");
 }

@@ -266,7 +266,9 @@ class Dft:
     """items: dicts q, b, rows, inp (tensor), in_stride, in_row_stride, out (tensor [rows][q^b]).
     rows4: q = 4, b in {4, 6} contiguous rows go to the whole-row kernel."""
 
-    def __init__(self, items, precision, rows4=True):
+    def __init__(self, items, precision, rows4=True, candidate=None):
+        """candidate: use the planner's schedule number `candidate` for the
+        pass kernels instead of the tuned / model-best one."""
         groups = defaultdict(list)
         fast = defaultdict(list)
         for it in items:
@@ -288,7 +290,11 @@ class Dft:
                 d = L.DftDesc(q=q, b=it["b"], rows=it["rows"], in_stride=it.get("in_stride", 1),
                               in_row_stride=it.get("in_row_stride", q ** it["b"]), in_=_ptr(it["inp"]),
                               out=_ptr(it["out"]))
-                L.check(lib.qsft_dft_plan(ctypes.byref(d), precision), "qsft_dft_plan")
+                if candidate is None:
+                    L.check(lib.qsft_dft_plan(ctypes.byref(d), precision), "qsft_dft_plan")
+                else:
+                    L.check(lib.qsft_dft_plan_candidate(ctypes.byref(d), precision, int(candidate)),
+                            "qsft_dft_plan_candidate")
                 descs.append(d)
             npass = max(d.n_pass for d in descs)
             pl = [[d.pass_blocks[p] if p < d.n_pass else 0 for d in descs] for p in range(npass)]
@@ -308,10 +314,41 @@ class Dft:
                         "qsft_dft_pass")
 
 
-def dft(items, precision):
-    p = Dft(items, precision)
+def dft(items, precision, candidate=None, rows4=True):
+    p = Dft(items, precision, rows4=rows4, candidate=candidate)
     p.launch()
     return p
+
+
+_tuned = {}
+
+
+def dft_autotune(q, b, precision, rows, max_candidates=6, device=None):
+    """Time the planner's best-ranked DFT schedules for (q, b, precision) on
+    `rows` scratch rows (once per process and shape) and record the fastest;
+    returns its time in ms. Later plans of that shape use it."""
+    key = (q, b, precision)
+    if key in _tuned:
+        return _tuned[key]
+    elem = 8 if precision == L.QSFT_F32 else 16
+    rows = max(1, min(int(rows), (256 << 20) // (elem * q ** b) or 1))
+    nbytes = (rows * q ** b * elem + 255) // 256 * 256 + 4096
+    dev = device if device is not None else torch.device("cuda", torch.cuda.current_device())
+    buf = torch.zeros(nbytes, dtype=torch.uint8, device=dev)
+    best = ctypes.c_float()
+    L.check(lib.qsft_dft_autotune(q, b, precision, rows, _vp(buf.data_ptr()), nbytes, int(max_candidates),
+                                  stream_handle(), ctypes.byref(best)), "qsft_dft_autotune")
+    _tuned[key] = float(best.value)
+    return _tuned[key]
+
+
+def dft_schedule(q, b, precision):
+    """(passes [(d0, d1, ti, ta)], tuned) the planner uses for (q, b, precision)."""
+    n = ctypes.c_int()
+    arrs = [(ctypes.c_int * 4)() for _ in range(4)]
+    tuned = ctypes.c_int()
+    L.check(lib.qsft_dft_schedule(q, b, precision, ctypes.byref(n), *arrs, ctypes.byref(tuned)), "qsft_dft_schedule")
+    return [tuple(a[i] for a in arrs) for i in range(n.value)], bool(tuned.value)
 
 
 # ---------------------------------------------------------------- get_MDU step

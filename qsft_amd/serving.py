@@ -214,6 +214,18 @@ class Server:
         self.completed = []
         self.batches = 0
 
+    def tune(self, specs, max_candidates=6):
+        """Autotune the sub-DFT schedule of every (q, b) among `specs` not
+        tuned yet (FP32, C * P rows of the first request of each shape), before
+        serving them: tuning times kernels synchronously."""
+        from . import kernels as K
+        seen = {}
+        for sp in specs:
+            if not (sp.q == 4 and sp.b in (4, 6)):                # those use the whole-row kernel
+                seen.setdefault((sp.q, sp.b), sp.C * sp.P)
+        for (q, b), rows in seen.items():
+            K.dft_autotune(q, b, L.QSFT_F32, rows, max_candidates=max_candidates, device=self.device)
+
     def submit(self, spec, t_submit=None):
         rs = request_struct(spec, self.max_iter)
         fp = int(lib.qsft_request_footprint(ctypes.byref(rs), self.max_iter))

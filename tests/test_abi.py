@@ -42,12 +42,13 @@ def test_dft_plan_covers_every_axis_once(q, b, prec, passes):
     axes = []
     elem = 8 if prec == L.QSFT_F32 else 16
     for p in range(d.n_pass):
-        d0, d1, ti = d.pass_d0[p], d.pass_d1[p], d.pass_ti[p]
+        d0, d1, ti, ta = d.pass_d0[p], d.pass_d1[p], d.pass_ti[p], d.pass_ta[p]
         axes += list(range(d0, d1))
         inner = q ** (b - d1)
-        assert inner % ti == 0
-        assert q ** (d1 - d0) * ti * elem <= 65536 - 64 * 16
-        assert d.pass_blocks[p] == 5 * q ** d0 * (inner // ti)
+        assert inner % ti == 0 and (ta == 1 or ti == inner) and q ** d0 % ta == 0
+        assert q ** (d1 - d0) * ti * ta * elem <= 65536 - 64 * 16
+        assert d.pass_blocks[p] == 5 * (q ** d0 // ta) * (inner // ti)
+        assert 0 < L.lib.qsft_dft_lds_bytes(ctypes.byref(d), p, prec) <= 65536
     assert sorted(axes) == list(range(b))
 
 

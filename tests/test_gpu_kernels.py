@@ -4,6 +4,8 @@ Tolerances: integer/index outputs bit-exact; FP64 samples and transforms
 within 1e-12 relative (max-abs over max |value|); FP32 within 2e-6 relative
 (FP32 accumulation of up to S terms plus the w-table rounding).
 """
+import ctypes
+
 import numpy as np
 import pytest
 import torch
@@ -13,6 +15,7 @@ from oracle import qsft_oracle as O
 pytestmark = pytest.mark.gpu
 
 if torch.cuda.is_available():
+    from qsft_amd import _lib as L
     from qsft_amd import device as dv
     from qsft_amd import kernels as K
 
@@ -203,7 +206,58 @@ def test_dft_whole_row_kernel_matches_pass_kernel(b, precision):
     ref = np.stack([O.gwht(r, q, b) for r in x])
     tol = 1e-13 if precision == "fp64" else 3e-6
     assert rel_err(outs[0], ref) < tol and rel_err(outs[1], ref) < tol
-    assert rel_err(outs[0], outs[1]) < tol
+    # both apply the four-point butterflies innermost digit first: the same bits
+    fl = np.float64 if precision == "fp64" else np.float32
+    assert np.array_equal(outs[0].view(fl), outs[1].view(fl))
+
+
+@pytest.mark.parametrize("q,b", [(20, 4), (20, 3), (4, 7), (4, 5), (3, 8), (2, 12), (5, 5), (7, 3)])
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_dft_every_schedule_gives_the_same_bits(q, b, precision):
+    """The planner's candidate schedules (different pass splits, tile shapes,
+    LDS budgets) apply the same axis transforms in the same order, so they
+    give the same bits -- and match the DFT definition. The q = 20 transform
+    is the prime-factor 4 x 5 one; 3 and 5 are the symmetric-pair forms."""
+    rows = 3
+    print(f"dft schedules q={q} b={b} {precision}", flush=True)
+    rng = np.random.default_rng(q * 31 + b)
+    x = rng.normal(size=(rows, q ** b)) + 1j * rng.normal(size=(rows, q ** b))
+    dt = torch.complex128 if precision == "fp64" else torch.complex64
+    code = dv.PRECISIONS[precision][0]
+    dev = dv.current_device()
+    outs = []
+    for cand in range(6):
+        d = L.DftDesc(q=q, b=b, rows=rows, in_stride=1, in_row_stride=q ** b)
+        if L.lib.qsft_dft_plan_candidate(ctypes.byref(d), code, cand) != L.QSFT_OK:
+            break
+        xt = torch.from_numpy(x).to(dt).to(dev)
+        yt = torch.empty_like(xt)
+        K.Dft([dict(q=q, b=b, rows=rows, inp=xt, out=yt)], code, rows4=False, candidate=cand).launch()
+        outs.append(yt.cpu().numpy())
+    assert len(outs) >= 2
+    ref = np.stack([O.gwht(r, q, b) for r in x])
+    assert rel_err(outs[0], ref) < (1e-13 if precision == "fp64" else 3e-6)
+    for o in outs[1:]:
+        assert np.array_equal(o.view(np.float64 if precision == "fp64" else np.float32),
+                              outs[0].view(np.float64 if precision == "fp64" else np.float32))
+
+
+def test_dft_autotune_records_the_fastest_schedule():
+    """qsft_dft_autotune times the candidates on the device and records the
+    winner, which later plans of the shape use (same bits as candidate 0)."""
+    q, b, rows = 20, 4, 4
+    before, tuned0 = K.dft_schedule(q, b, L.QSFT_F32)
+    ms = K.dft_autotune(q, b, L.QSFT_F32, rows)
+    after, tuned1 = K.dft_schedule(q, b, L.QSFT_F32)
+    print(f"q=20 b=4: model {before} -> tuned {after} ({ms:.3f} ms for {rows} rows)", flush=True)
+    assert not tuned0 and tuned1 and ms > 0
+    rng = np.random.default_rng(3)
+    x = rng.normal(size=(rows, q ** b)) + 1j * rng.normal(size=(rows, q ** b))
+    xt = torch.from_numpy(x).to(torch.complex64).to(dv.current_device())
+    got = dv.subtransform(xt, q, b, precision="fp32").cpu().numpy()
+    y0 = torch.empty_like(xt)
+    K.Dft([dict(q=q, b=b, rows=rows, inp=xt, out=y0)], L.QSFT_F32, rows4=False, candidate=0).launch()
+    assert np.array_equal(got.view(np.float32), y0.cpu().numpy().view(np.float32))
 
 
 @pytest.mark.parametrize("q,b,bb", [(4, 6, 4), (3, 5, 2), (20, 3, 2)])
