@@ -233,6 +233,9 @@ int qsft_assemble_blocks(const qsft_assemble_desc* host_desc);
  * instances. QSFT_ASM_F32_TILE: FP32 source and no noise array, so the
  * bin-major tile may be held in FP32 (half the LDS per workgroup). */
 #define QSFT_ASM_F32_TILE 1
+/* Launch word bit: seed_offset holds the device address of a uint64 offset,
+ * read when the kernel runs (a captured graph replayed with fresh noise). */
+#define QSFT_ASM_SEED_PTR 2
 int qsft_assemble_word(const qsft_assemble_desc* host_desc);
 /* seed_offset is added to every descriptor's Philox seed, so one uploaded
  * batch can draw fresh noise on every launch. */
@@ -448,6 +451,13 @@ int qsft_batch_delay_sign(void* batch, int i);
 int qsft_batch_bind(void* batch, void* input_dev, void* work_dev, void* plan_host, void* plan_dev, void* out);
 int qsft_batch_launch(void* batch, int upload, const void* input_host, uint64_t seed_offset, void* stream,
                       void* tail_stream);
+/* hipGraph of the batch's relaunch (no upload): the whole pipeline, tail
+ * stream included, captured once on `stream` (after a launch that uploaded
+ * the inputs; waits for it). Each replay first bumps a device counter that
+ * the assembler adds to the Philox seeds, so replay k draws the noise of an
+ * eager qsft_batch_launch with seed_offset = k (k = 1, 2, ... since capture). */
+int qsft_batch_capture(void* batch, void* stream, void* tail_stream);
+int qsft_batch_replay(void* batch, void* stream);
 /* 1: the last launch's rows are in `out`; 0: not yet; < 0: error. */
 int qsft_batch_done(void* batch);
 int qsft_batch_wait(void* batch);

@@ -143,6 +143,8 @@ SIGNATURES = {
     "qsft_batch_delay_sign": (ctypes.c_int, [vp, ctypes.c_int]),
     "qsft_batch_bind": (ctypes.c_int, [vp, vp, vp, vp, vp, vp]),
     "qsft_batch_launch": (ctypes.c_int, [vp, ctypes.c_int, vp, ctypes.c_uint64, vp, vp]),
+    "qsft_batch_capture": (ctypes.c_int, [vp, vp, vp]),
+    "qsft_batch_replay": (ctypes.c_int, [vp, vp]),
     "qsft_batch_done": (ctypes.c_int, [vp]),
     "qsft_batch_wait": (ctypes.c_int, [vp]),
     "qsft_batch_free": (None, [vp]),

@@ -51,7 +51,7 @@ __device__ __forceinline__ double2 to_d2(float2 v) { return make_double2(v.x, v.
 template <typename TT>
 __global__ __launch_bounds__(kAsmThreads) void k_assemble(const qsft_assemble_desc* __restrict__ descs,
                                                           const int32_t* __restrict__ pre, int n_inst,
-                                                          uint64_t seed_offset) {
+                                                          uint64_t seed_offset, const uint64_t* seed_ptr) {
     __shared__ TT tile[kBmRows][kBmBins + 1];
     __shared__ int64_t srow[kBmRows];
     const int inst = find_inst(pre, n_inst, blockIdx.x);
@@ -60,7 +60,7 @@ __global__ __launch_bounds__(kAsmThreads) void k_assemble(const qsft_assemble_de
     const int rows = d.R * d.p1;
     const int local = blockIdx.x - pre[inst];
     const double comp_sd = d.sigma * 0.70710678118654752440;      // per component: sqrt(sigma^2 / 2)
-    const uint64_t seed = d.seed + seed_offset;
+    const uint64_t seed = d.seed + (seed_ptr ? *seed_ptr : seed_offset);
     if (d.seed_out && local == 0 && threadIdx.x == 0) *d.seed_out = seed;
     if (!d.bin_major) {
         const int64_t total = (int64_t)d.C * rows * Q;
@@ -232,16 +232,20 @@ extern "C" int qsft_assemble_word(const qsft_assemble_desc* h) {
 
 extern "C" int qsft_assemble(const qsft_assemble_desc* d, const int32_t* pre, int n_inst, int blocks,
                              uint64_t seed_offset, int word, void* stream) {
-    if (!d || !pre || n_inst < 1 || blocks < 0 || (word & ~QSFT_ASM_F32_TILE)) {
+    if (!d || !pre || n_inst < 1 || blocks < 0 || (word & ~(QSFT_ASM_F32_TILE | QSFT_ASM_SEED_PTR)) ||
+        ((word & QSFT_ASM_SEED_PTR) && !seed_offset)) {
         qsft_set_error("qsft_assemble: bad arguments");
         return QSFT_ERR_ARG;
     }
     if (blocks == 0) return QSFT_OK;
+    // QSFT_ASM_SEED_PTR: seed_offset is the device address of the offset (read by the kernel)
+    const uint64_t* sp = (word & QSFT_ASM_SEED_PTR) ? reinterpret_cast<const uint64_t*>(seed_offset) : nullptr;
+    const uint64_t so = sp ? 0 : seed_offset;
     if (word & QSFT_ASM_F32_TILE)
         hipLaunchKernelGGL(k_assemble<float2>, dim3(blocks), dim3(kAsmThreads), 0, (hipStream_t)stream, d, pre,
-                           n_inst, seed_offset);
+                           n_inst, so, sp);
     else
         hipLaunchKernelGGL(k_assemble<double2>, dim3(blocks), dim3(kAsmThreads), 0, (hipStream_t)stream, d, pre,
-                           n_inst, seed_offset);
+                           n_inst, so, sp);
     return qsft_hip_status(hipGetLastError(), "qsft_assemble");
 }

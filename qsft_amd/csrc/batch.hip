@@ -162,8 +162,16 @@ struct Batch {
     int peel_bins = 0, peel_slots = 0;
     size_t stage_desc = 0, stage_pre = 0;
     int stage_blocks = 0;
+    size_t seed_ctr = 0;                    // plan offset of the replay seed counter
     hipEvent_t fork = nullptr, done = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
 };
+
+// One replay of a captured batch: the next seed offset.
+__global__ void k_seed_bump(uint64_t* ctr) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *ctr = *ctr + 1;
+}
 
 // Appends descriptor arrays and block prefixes to the plan; with `write`,
 // into plan_host (bases are the real arenas), else only sizes (fake bases).
@@ -201,6 +209,10 @@ int build_plan(Batch& B, bool write) {
     auto in_ptr = [&](int i, size_t off) { return (void*)(in_base + B.lay[i].in_off + off); };
     auto work_ptr = [&](int i, size_t off) { return (void*)(work_base + B.lay[i].work_off + off); };
 
+    {
+        const uint64_t zero = 0;
+        B.seed_ctr = W.put(&zero, sizeof(zero));                   // replay seed counter, starts at 0
+    }
     // ---- support digits: locq [n][S] -> K [S][n_pad] on the device
     {
         std::vector<StageDesc> sd(nreq);
@@ -550,16 +562,15 @@ extern "C" int qsft_batch_bind(void* h, void* input_dev, void* work_dev, void* p
     return rc;
 }
 
-extern "C" int qsft_batch_launch(void* h, int upload, const void* input_host, uint64_t seed_offset, void* stream,
-                                 void* tail_stream) {
-    Batch* B = as_batch(h);
-    if (!B || !B->bound || (upload & ~3) || ((upload & 1) && !input_host)) {
-        qsft_set_error("qsft_batch_launch: batch not bound or bad upload flags");
-        return QSFT_ERR_ARG;
-    }
-    hipStream_t st = (hipStream_t)stream;
+namespace {
+
+// Enqueue the pipeline. seed_ptr (device) replaces seed_offset when set;
+// `capturing`: inside a stream capture (no wait on an event recorded outside
+// it; the tail stream is joined back to `stream` before the capture ends).
+int enqueue(Batch* B, int upload, const void* input_host, uint64_t seed_offset, const uint64_t* seed_ptr,
+            hipStream_t st, hipStream_t tail, bool capturing) {
     hipError_t e;
-    if (B->launched) {                                              // the last tail still reads the tables
+    if (B->launched && !capturing) {                               // the last tail still reads the tables
         e = hipStreamWaitEvent(st, B->done, 0);
         if (e != hipSuccess) return qsft_hip_status(e, "qsft_batch_launch(wait)");
     }
@@ -596,8 +607,12 @@ extern "C" int qsft_batch_launch(void* h, int upload, const void* input_host, ui
             if (rc != QSFT_OK) return rc;
         }
     const int n = (int)B->req.size();
-    rc = qsft_assemble((const qsft_assemble_desc*)(P + B->asm_desc), (const int32_t*)(P + B->asm_pre), n,
-                       B->asm_blocks, seed_offset, B->asm_word, st);
+    if (seed_ptr)
+        rc = qsft_assemble((const qsft_assemble_desc*)(P + B->asm_desc), (const int32_t*)(P + B->asm_pre), n,
+                           B->asm_blocks, (uint64_t)(uintptr_t)seed_ptr, B->asm_word | QSFT_ASM_SEED_PTR, st);
+    else
+        rc = qsft_assemble((const qsft_assemble_desc*)(P + B->asm_desc), (const int32_t*)(P + B->asm_pre), n,
+                           B->asm_blocks, seed_offset, B->asm_word, st);
     if (rc != QSFT_OK) return rc;
     const qsft_peel_desc* pd = (const qsft_peel_desc*)(P + B->peel_desc);
     const int32_t* pb = (const int32_t*)(P + B->peel_pre_bins);
@@ -608,8 +623,8 @@ extern "C" int qsft_batch_launch(void* h, int upload, const void* input_host, ui
     if (rc == QSFT_OK) rc = qsft_peel_passes(pd, n, B->max_p, B->fuse_from, B->max_iter, st);
     if (rc != QSFT_OK) return rc;
     hipStream_t ts = st;
-    if (tail_stream && tail_stream != stream) {
-        ts = (hipStream_t)tail_stream;
+    if (tail && tail != st) {
+        ts = tail;
         e = hipEventRecord(B->fork, st);
         if (e == hipSuccess) e = hipStreamWaitEvent(ts, B->fork, 0);
         if (e != hipSuccess) return qsft_hip_status(e, "qsft_batch_launch(fork)");
@@ -618,9 +633,66 @@ extern "C" int qsft_batch_launch(void* h, int upload, const void* input_host, ui
     if (rc == QSFT_OK) rc = qsft_peel_pack_ordered(pd, n, B->out, B->sz.cap_rows, B->sz.row_stride, ts);
     if (rc != QSFT_OK) return rc;
     e = hipEventRecord(B->done, ts);
+    if (e == hipSuccess && capturing && ts != st) e = hipStreamWaitEvent(st, B->done, 0);   // join the tail
     if (e != hipSuccess) return qsft_hip_status(e, "qsft_batch_launch(done)");
-    B->launched = true;
     return QSFT_OK;
+}
+
+}  // namespace
+
+extern "C" int qsft_batch_launch(void* h, int upload, const void* input_host, uint64_t seed_offset, void* stream,
+                                 void* tail_stream) {
+    Batch* B = as_batch(h);
+    if (!B || !B->bound || (upload & ~3) || ((upload & 1) && !input_host)) {
+        qsft_set_error("qsft_batch_launch: batch not bound or bad upload flags");
+        return QSFT_ERR_ARG;
+    }
+    const int rc = enqueue(B, upload, input_host, seed_offset, nullptr, (hipStream_t)stream, (hipStream_t)tail_stream,
+                           false);
+    if (rc == QSFT_OK) B->launched = true;
+    return rc;
+}
+
+extern "C" int qsft_batch_capture(void* h, void* stream, void* tail_stream) {
+    Batch* B = as_batch(h);
+    if (!B || !B->bound || !B->launched || !stream) {
+        qsft_set_error("qsft_batch_capture: bind and launch the batch (with its upload) on a stream first");
+        return QSFT_ERR_ARG;
+    }
+    hipError_t e = hipEventSynchronize(B->done);                   // inputs staged, last launch finished
+    if (e != hipSuccess) return qsft_hip_status(e, "qsft_batch_capture(sync)");
+    if (B->graph_exec) { (void)hipGraphExecDestroy(B->graph_exec); B->graph_exec = nullptr; }
+    if (B->graph) { (void)hipGraphDestroy(B->graph); B->graph = nullptr; }
+    hipStream_t st = (hipStream_t)stream;
+    e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return qsft_hip_status(e, "qsft_batch_capture(begin)");
+    uint64_t* ctr = reinterpret_cast<uint64_t*>(B->plan_dev + B->seed_ctr);
+    hipLaunchKernelGGL(k_seed_bump, dim3(1), dim3(64), 0, st, ctr);
+    int rc = qsft_hip_status(hipGetLastError(), "qsft_batch_capture(bump)");
+    if (rc == QSFT_OK) rc = enqueue(B, 0, nullptr, 0, ctr, st, (hipStream_t)tail_stream, true);
+    hipGraph_t g = nullptr;
+    e = hipStreamEndCapture(st, &g);
+    if (rc != QSFT_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return qsft_hip_status(e, "qsft_batch_capture(end)");
+    B->graph = g;
+    e = hipGraphInstantiate(&B->graph_exec, g, nullptr, nullptr, 0);
+    return qsft_hip_status(e, "qsft_batch_capture(instantiate)");
+}
+
+extern "C" int qsft_batch_replay(void* h, void* stream) {
+    Batch* B = as_batch(h);
+    if (!B || !B->graph_exec || !stream) {
+        qsft_set_error("qsft_batch_replay: no captured graph");
+        return QSFT_ERR_ARG;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    hipError_t e = hipStreamWaitEvent(st, B->done, 0);
+    if (e == hipSuccess) e = hipGraphLaunch(B->graph_exec, st);
+    if (e == hipSuccess) e = hipEventRecord(B->done, st);
+    return qsft_hip_status(e, "qsft_batch_replay");
 }
 
 extern "C" int qsft_batch_done(void* h) {
@@ -650,6 +722,8 @@ extern "C" void qsft_batch_free(void* h) {
     Batch* B = as_batch(h);
     if (!B) return;
     if (B->launched) (void)hipEventSynchronize(B->done);
+    if (B->graph_exec) (void)hipGraphExecDestroy(B->graph_exec);
+    if (B->graph) (void)hipGraphDestroy(B->graph);
     if (B->fork) (void)hipEventDestroy(B->fork);
     if (B->done) (void)hipEventDestroy(B->done);
     delete B;

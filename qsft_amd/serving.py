@@ -127,6 +127,20 @@ class ServedBatch:
                                       st, ts), "qsft_batch_launch")
         self.uploaded = True
 
+    def capture(self):
+        """Capture the batch's relaunch (the whole pipeline, tail stream
+        included) as a hipGraph; needs a dedicated stream and one launch
+        first. Replay k draws the noise of launch(seed_offset=k)."""
+        if self.stream is None:
+            raise ValueError("graph capture needs the batch's own stream (not the default stream)")
+        ts = self.tail_stream.cuda_stream if self.tail_stream is not None else None
+        L.check(lib.qsft_batch_capture(self.h, self.stream.cuda_stream, ts), "qsft_batch_capture")
+        self.replays = 0
+
+    def replay(self):
+        L.check(lib.qsft_batch_replay(self.h, self.stream.cuda_stream), "qsft_batch_replay")
+        self.replays += 1
+
     def done(self):
         rc = lib.qsft_batch_done(self.h)
         if rc < 0:
