@@ -40,6 +40,10 @@ extern "C" {
 enum { QSFT_OK = 0, QSFT_ERR_ARG = 1, QSFT_ERR_HIP = 2, QSFT_ERR_DEVICE = 3 };
 enum { QSFT_F32 = 0, QSFT_F64 = 1 };
 enum { QSFT_DETECT_NOISELESS = 0, QSFT_DETECT_NSO1 = 1 };
+/* method bit: coded (Reed-Solomon) source delays; the channel detection
+ * (NOISELESS or NSO1) yields p1 - 1 = 2 t s syndrome digits, decoded into k
+ * by the device syndrome decoder (qsft_peel_set_coded supplies the code). */
+#define QSFT_DETECT_CODED 2
 enum { QSFT_NOISE_NONE = 0, QSFT_NOISE_ARRAY = 1, QSFT_NOISE_PHILOX = 2 };
 
 /* device-side error bits reported in qsft_peel_desc::ctr[QSFT_CTR_ERROR] */
@@ -310,6 +314,10 @@ typedef struct qsft_peel_desc {
     double noise_sigma;     /* the assembler's sigma (0: no noise)                          */
     uint8_t* written;       /* [C*Q] 1: the bin's column is in U (workspace)                */
     uint64_t* noise_key;    /* the assembler's Philox key of this launch (workspace)        */
+    /* coded source (method & QSFT_DETECT_CODED), set by qsft_peel_set_coded */
+    const int16_t* rs_tab;  /* GF(q^s) tables: exp[2 (q^s - 1)] then log[q^s] (log 0 = -1)   */
+    int32_t rs_t, rs_s;     /* correctable errors t, symbol digits s (p1 - 1 = 2 t s)        */
+    int32_t rs_nc, _pad1;   /* code length q^s - 1                                          */
 } qsft_peel_desc;
 
 /* qsft_peel_desc::flags */
@@ -340,6 +348,15 @@ int qsft_peel_bind(qsft_peel_desc* host_desc, int q, int n, int b, int C, int P,
                    int method, int max_iter, double cutoff, int out_cap,
                    const int8_t* Mt, const int8_t* D, double* U,
                    void* workspace, size_t workspace_bytes, int n_pad);
+/* Coded source delays (qsft/query.py:88-92, qsft/reconstruct.py:34-51,
+ * qsft/ReedSolomon.py:27-50): the shortened Reed-Solomon code over GF(q^s)
+ * whose parity-check rows H[j][i] = alpha^{(1+j)(n-1-i)} give the delay rows.
+ * The detection kernels decode each bin's 2t syndrome symbols by
+ * Berlekamp-Massey, a Chien search over all q^s - 1 positions and Forney's
+ * formula; a failed decode (locator roots != its degree, or an error value
+ * outside GF(q)) gives k = 0. `tables` (device) must stay valid while the
+ * descriptor is used. Requires method & QSFT_DETECT_CODED and p1 - 1 = 2 t s. */
+int qsft_peel_set_coded(qsft_peel_desc* host_desc, int t, int s, const int16_t* tables);
 int qsft_peel_bin_blocks(const qsft_peel_desc* host_desc);   /* blocks of the per-bin kernels  */
 int qsft_peel_slot_blocks(const qsft_peel_desc* host_desc);  /* blocks of the per-slot kernels */
 

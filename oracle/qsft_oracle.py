@@ -123,6 +123,178 @@ def get_Ms(n, b, q, num_to_get=None, method="simple"):
     raise NotImplementedError(method)
 
 
+# ---------------------------------------------------------------------------
+# Coded source delays: Reed-Solomon over GF(q^s) (qsft/ReedSolomon.py,
+# qsft/query.py:88-92,231-243, qsft/reconstruct.py:34-51). The reference takes
+# the code from `galois` (absent here); this restates the construction with
+# plain polynomial arithmetic: GF(q)[x] / f, f the smallest primitive
+# polynomial (galois's default is the Conway polynomial -- parity of the
+# delay matrix with the reference is unpinned), H[j, i] =
+# alpha^{(1+j)(n-1-i)}, digits of each entry highest degree first. The
+# decoder is Peterson-Gorenstein-Zierler (locator by linear solves over
+# GF(q^s)), roots by exhaustive search over all code positions and error
+# values by solving the Vandermonde system -- a different algorithm from
+# the engine's Berlekamp-Massey + Forney.
+# ---------------------------------------------------------------------------
+
+
+class OracleGF:
+    """GF(q^s) elements as coefficient tuples (c_0 .. c_{s-1}, low degree first)."""
+
+    def __init__(self, q, s):
+        self.q, self.s = q, s
+        self.f = None
+        for v in range(q ** s):                              # candidates ordered by c_{s-1} .. c_0
+            c = [(v // q ** i) % q for i in range(s)]
+            if c[0] == 0:
+                continue
+            self.f = c
+            if self._order_of_x() == q ** s - 1:
+                break
+            self.f = None
+        if self.f is None:
+            raise ValueError("no primitive polynomial")
+        self.zero = tuple([0] * s)
+        self.one = tuple([1] + [0] * (s - 1))
+
+    def _mulx(self, a):
+        a = list(a)
+        top = a[-1]
+        a = [0] + a[:-1]
+        return tuple((a[i] - top * self.f[i]) % self.q for i in range(self.s))
+
+    def _order_of_x(self):
+        a = self._mulx(tuple([1] + [0] * (self.s - 1)))
+        k = 1
+        while a != tuple([1] + [0] * (self.s - 1)):
+            a = self._mulx(a)
+            k += 1
+            if k > self.q ** self.s:
+                return -1
+        return k
+
+    def add(self, a, b):
+        return tuple((x + y) % self.q for x, y in zip(a, b))
+
+    def sub(self, a, b):
+        return tuple((x - y) % self.q for x, y in zip(a, b))
+
+    def mul(self, a, b):
+        acc = self.zero
+        cur = a
+        for bi in b:                                         # sum_i b_i x^i a
+            if bi:
+                acc = self.add(acc, tuple((bi * c) % self.q for c in cur))
+            cur = self._mulx(cur)
+        return acc
+
+    def power(self, a, e):
+        r = self.one
+        for _ in range(e):
+            r = self.mul(r, a)
+        return r
+
+    def inv(self, a):
+        return self.power(a, self.q ** self.s - 2)
+
+    def alpha_pow(self, e):
+        return self.power(tuple([0, 1] + [0] * (self.s - 2)) if self.s > 1 else (self._x1(),), e % (self.q ** self.s - 1))
+
+    def _x1(self):
+        # s = 1: alpha is the root of x - g, i.e. the element g = -c_0
+        return (-self.f[0]) % self.q
+
+    def digits_msb(self, a):
+        return list(reversed(a))
+
+    def from_digits_msb(self, d):
+        return tuple(int(x) % self.q for x in reversed(list(d)))
+
+
+def rs_params(n, t, q):
+    """s, code length q^s - 1 (qsft/ReedSolomon.py:18-25)."""
+    s = int(np.ceil(np.log(n) / np.log(q)))
+    if n > q ** s - 1:
+        s += 1
+    return s, q ** s - 1
+
+
+def rs_delay_matrix(n, t, q):
+    """(2 t s + 1) x n coded source delays (qsft/ReedSolomon.py:52-65)."""
+    s, nc = rs_params(n, t, q)
+    F = OracleGF(q, s)
+    D = np.zeros((2 * t * s + 1, n), dtype=np.int64)
+    for j in range(2 * t):
+        for i in range(n):
+            D[s * j + 1: s * (j + 1) + 1, i] = F.digits_msb(F.alpha_pow((1 + j) * (n - 1 - i)))
+    return D
+
+
+def _gf_solve(F, A, y):
+    """Solve A x = y over GF(q^s) by Gauss-Jordan; None if singular."""
+    m = len(A)
+    M = [list(row) + [y[i]] for i, row in enumerate(A)]
+    for col in range(m):
+        piv = next((r for r in range(col, m) if M[r][col] != F.zero), None)
+        if piv is None:
+            return None
+        M[col], M[piv] = M[piv], M[col]
+        iv = F.inv(M[col][col])
+        M[col] = [F.mul(iv, v) for v in M[col]]
+        for r in range(m):
+            if r != col and M[r][col] != F.zero:
+                f = M[r][col]
+                M[r] = [F.sub(a, F.mul(f, b)) for a, b in zip(M[r], M[col])]
+    return [M[i][m] for i in range(m)]
+
+
+def rs_decode(digits, n, t, q):
+    """k (length n) from the 2 t s syndrome digits by PGZ; (k, errors), errors = -1
+    and k = 0 when no error pattern of weight <= t in GF(q) has that syndrome."""
+    s, nc = rs_params(n, t, q)
+    F = OracleGF(q, s)
+    S = [F.from_digits_msb(digits[s * j: s * (j + 1)]) for j in range(2 * t)]     # S_1 .. S_2t
+    zero_k = np.zeros(n, dtype=np.int64)
+    if all(x == F.zero for x in S):
+        return zero_k, 0
+    for nu in range(t, 0, -1):
+        A = [[S[i + j] for j in range(nu)] for i in range(nu)]     # S_{i+j+1}
+        rhs = [F.sub(F.zero, S[nu + i]) for i in range(nu)]         # -S_{nu+i+1}
+        sol = _gf_solve(F, A, rhs)
+        if sol is None:
+            continue
+        lam = [F.one] + list(reversed(sol))                        # Lambda_1 .. Lambda_nu
+        locs = []
+        for pos in range(nc):
+            xinv = F.alpha_pow(-(nc - 1 - pos))
+            acc, pw = F.zero, F.one
+            for c in lam:
+                acc = F.add(acc, F.mul(c, pw))
+                pw = F.mul(pw, xinv)
+            if acc == F.zero:
+                locs.append(pos)
+        if len(locs) != nu:
+            return zero_k, -1
+        X = [F.alpha_pow(nc - 1 - p) for p in locs]
+        V = [[F.power(x, j + 1) for x in X] for j in range(nu)]     # S_j = sum_l e_l X_l^j
+        e = _gf_solve(F, V, S[:nu])
+        if e is None or any(any(c != 0 for c in v[1:]) for v in e):
+            return zero_k, -1
+        k = zero_k.copy()
+        for p, v in zip(locs, e):
+            if p >= nc - n:
+                k[p - (nc - n)] = v[0]
+        # the solution must reproduce every syndrome
+        for j in range(2 * t):
+            acc = F.zero
+            for x, v in zip(X, e):
+                acc = F.add(acc, F.mul(v, F.power(x, j + 1)))
+            if acc != S[j]:
+                return zero_k, -1
+        return k, nu
+    return zero_k, -1
+
+
 def get_D(n, **kw):
     """Delay matrices (qsft/query.py:73-143)."""
     q = kw.get("q")
@@ -131,6 +303,8 @@ def get_D(n, **kw):
         D = np.random.choice(q, (kw.get("num_delays"), n))
     elif source == "identity":
         D = np.vstack((np.zeros(n), np.eye(n))).astype(int)
+    elif source == "coded":
+        D = rs_delay_matrix(n, kw.get("t"), q)
     else:
         raise NotImplementedError(f"delays_method_source={source!r}")
     channel = kw.get("delays_method_channel", "identity")
@@ -326,15 +500,19 @@ def detect_nso1(col, q, p1):
     return k
 
 
-def singleton_detection(col, method_source, method_channel, q, source_parity):
-    """Dispatcher (reconstruct.py:132-167); only the identity source is in scope."""
+def singleton_detection(col, method_source, method_channel, q, source_parity, code=None):
+    """Dispatcher (reconstruct.py:132-167): channel decoding, then for coded
+    sources the syndrome decoding of reconstruct.py:34-51 (code = (n, t))."""
     if method_channel == "nso":
         k = detect_nso1(col, q, source_parity)
     elif method_channel == "identity":
         k = detect_noiseless(col, q)
     else:
         raise NotImplementedError(f"reconstruct_method_channel={method_channel!r}")
-    if method_source != "identity":
+    if method_source == "coded":
+        n, t = code
+        k, _ = rs_decode(k, n, t, q)
+    elif method_source != "identity":
         raise NotImplementedError(f"reconstruct_method_source={method_source!r}")
     return k
 
@@ -345,7 +523,7 @@ def singleton_detection(col, method_source, method_channel, q, source_parity):
 
 
 def peel_decode(Ms, Ds, Us, q, n, cutoff, method_source="identity", method_channel="nso",
-                source_parity=None, max_iter=15, trace=None):
+                source_parity=None, max_iter=15, trace=None, t=None):
     """The detection + peeling loop of QSFT.transform (qsft.py:142-255).
 
     Ms: list of C (n, b) ints; Ds: list of C (P, n) ints; Us: (C, P, Q)
@@ -371,7 +549,7 @@ def peel_decode(Ms, Ds, Us, q, n, cutoff, method_source="identity", method_chann
             U, M, D = Us[i], Ms[i], Ds[i]
             for j, col in enumerate(U.T):
                 if np.linalg.norm(col) ** 2 > cutoff * len(col):
-                    k = singleton_detection(col, method_source, method_channel, q, source_parity)
+                    k = singleton_detection(col, method_source, method_channel, q, source_parity, code=(n, t))
                     sig = omega ** (D @ k)
                     rho = np.dot(np.conjugate(sig), col) / D.shape[0]
                     res = col - rho * sig
@@ -413,7 +591,7 @@ def peel_decode(Ms, Ds, Us, q, n, cutoff, method_source="identity", method_chann
 
 
 def qsft_transform(signal, num_subsample, num_repeat, b, method_source="identity",
-                   method_channel="nso", report=False, cutoff=None, sort=False, trace=None):
+                   method_channel="nso", report=False, cutoff=None, sort=False, trace=None, t=None):
     """QSFT(...).transform(signal) (qsft.py:47-281) on an oracle signal."""
     q, n = signal.q, signal.n
     Ms, Ds, Us, Ts = signal.get_MDU(num_subsample, num_repeat, b, trans_times=True)
@@ -426,7 +604,8 @@ def qsft_transform(signal, num_subsample, num_repeat, b, method_source="identity
         cutoff = 1e-9 + 1.5 * (signal.noise_sd ** 2) / (q ** b)
     t0 = time.time()
     _, gw, _ = peel_decode(Ms, Ds, Us, q, n, cutoff, method_source, method_channel,
-                           signal.get_source_parity(), trace=trace)
+                           signal.get_source_parity(), trace=trace,
+                           t=t if t is not None else signal.query_args.get("t"))
     peel_time = time.time() - t0
     if not report:
         return gw

@@ -18,6 +18,7 @@ LIB_PATH = os.environ.get("QSFT_AMD_LIB", os.path.join(HERE, "lib", "libqsft_amd
 QSFT_OK, QSFT_ERR_ARG, QSFT_ERR_HIP, QSFT_ERR_DEVICE = 0, 1, 2, 3
 QSFT_F32, QSFT_F64 = 0, 1
 QSFT_DETECT_NOISELESS, QSFT_DETECT_NSO1 = 0, 1
+QSFT_DETECT_CODED = 2
 QSFT_NOISE_NONE, QSFT_NOISE_ARRAY, QSFT_NOISE_PHILOX = 0, 1, 2
 QSFT_DEVERR_TABLE_FULL, QSFT_DEVERR_TAG_COLLISION, QSFT_DEVERR_OUT_FULL = 1, 2, 4
 QSFT_MAX_Q, QSFT_MAX_N, QSFT_MAX_B, QSFT_MAX_C, QSFT_MAX_R = 64, 256, 16, 16, 16
@@ -70,7 +71,8 @@ class PeelDesc(ctypes.Structure):
                 ("last", vp), ("head", vp), ("count", vp), ("live", vp), ("key", vp), ("rec_time", vp),
                 ("rec_rho", vp), ("rec_next", vp), ("ctr", vp),
                 ("out_k", vp), ("out_val", vp), ("out_order", vp),
-                ("xsrc", vp), ("noise_sigma", f64), ("written", vp), ("noise_key", vp)]
+                ("xsrc", vp), ("noise_sigma", f64), ("written", vp), ("noise_key", vp),
+                ("rs_tab", vp), ("rs_t", i32), ("rs_s", i32), ("rs_nc", i32), ("_pad1", i32)]
 
 
 class EvalMfmaDesc(ctypes.Structure):
@@ -125,6 +127,7 @@ SIGNATURES = {
                                       ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                       ctypes.c_double, ctypes.c_int, vp, vp, vp, vp, ctypes.c_size_t,
                                       ctypes.c_int]),
+    "qsft_peel_set_coded": (ctypes.c_int, [ctypes.POINTER(PeelDesc), ctypes.c_int, ctypes.c_int, vp]),
     "qsft_peel_bin_blocks": (ctypes.c_int, [ctypes.POINTER(PeelDesc)]),
     "qsft_peel_slot_blocks": (ctypes.c_int, [ctypes.POINTER(PeelDesc)]),
     "qsft_peel_run": (ctypes.c_int, [vp, vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,

@@ -589,9 +589,155 @@ __device__ __forceinline__ uint32_t noiseless_digit(double2 v0, double2 vi, int 
 
 // Per-wave LDS of the detection: the k digits, two column slots with the
 // first ball queued on each, and the singletons buffered for registration.
+// ---- coded source: Reed-Solomon syndrome decoding in one wave
+constexpr int kRsMaxSyn = 128;                                     // syndrome digits 2 t s
+constexpr int kRsMaxT2 = 32;                                       // 2 t
+constexpr int kRsScratch = 168;                                    // S[32] lam[33] B[33] omega[32] T[33] L status
+
+// GF(q^s): an element is the integer whose base-q digits are its polynomial
+// coefficients; exp/log tables of the primitive element alpha = x.
+struct GFDev {
+    const int16_t* ex;
+    const int16_t* lg;
+    int q, s, nc;
+    __device__ int add(int a, int b) const {
+        int r = 0, p = 1;
+        for (int i = 0; i < s; ++i) {
+            int x = a % q + b % q;
+            if (x >= q) x -= q;
+            r += x * p;
+            a /= q;
+            b /= q;
+            p *= q;
+        }
+        return r;
+    }
+    __device__ int neg(int a) const {
+        int r = 0, p = 1;
+        for (int i = 0; i < s; ++i) {
+            const int x = a % q;
+            r += (x ? q - x : 0) * p;
+            a /= q;
+            p *= q;
+        }
+        return r;
+    }
+    __device__ int mul(int a, int b) const { return (a == 0 || b == 0) ? 0 : (int)ex[lg[a] + lg[b]]; }
+    __device__ int inv(int a) const { return (int)ex[nc - lg[a]]; }
+    __device__ int alpha(int e) const {
+        e %= nc;
+        if (e < 0) e += nc;
+        return (int)ex[e];
+    }
+};
+
+// k from the 2t syndrome symbols (s digits each, highest degree first) in
+// `syn`: Berlekamp-Massey on lane 0 (LDS state in `rs`), then a lane-parallel
+// Chien search over the q^s - 1 code positions with Forney's error values
+// e = -Omega(X^-1) / Lambda'(X^-1). Positions past the shortened part hold
+// k. A failed decode (locator degree > t, roots != degree, a zero
+// derivative, an error value outside GF(q)) leaves k = 0, as the received
+// all-zero word. kb8 (n_pad bytes, zero on entry) receives k.
+__device__ __noinline__ void rs_decode_wave(const qsft_peel_desc& d, const int8_t* syn, int8_t* kb8, int32_t* rs,
+                                            int lane) {
+    const int t2 = 2 * d.rs_t, s = d.rs_s, nc = d.rs_nc, q = d.q, n = d.n;
+    if (!d.rs_tab || t2 < 2 || t2 > kRsMaxT2) return;              // no code given: k = 0
+    const GFDev F{d.rs_tab, d.rs_tab + 2 * nc, q, s, nc};
+    int32_t* S = rs;
+    int32_t* lam = rs + 32;
+    int32_t* bp = rs + 65;
+    int32_t* om = rs + 98;
+    int32_t* tmp = rs + 130;
+    if (lane == 0) {
+        bool any = false;
+        for (int j = 0; j < t2; ++j) {
+            int v = 0;
+            for (int r = 0; r < s; ++r) v = v * q + (int)syn[s * j + r];
+            S[j] = v;
+            any = any || v != 0;
+        }
+        int status = 0, L = 0;
+        if (any) {
+            for (int i = 0; i <= t2; ++i) lam[i] = bp[i] = (i == 0);
+            int m = 1, bd = 1;
+            for (int r = 0; r < t2; ++r) {
+                int disc = S[r];
+                for (int i = 1; i <= L; ++i) disc = F.add(disc, F.mul(lam[i], S[r - i]));
+                if (disc == 0) {
+                    ++m;
+                    continue;
+                }
+                const int coef = F.mul(disc, F.inv(bd));
+                if (2 * L <= r) {
+                    for (int i = 0; i <= t2; ++i) tmp[i] = lam[i];
+                    for (int i = 0; i + m <= t2; ++i) lam[i + m] = F.add(lam[i + m], F.neg(F.mul(coef, bp[i])));
+                    for (int i = 0; i <= t2; ++i) bp[i] = tmp[i];
+                    L = r + 1 - L;
+                    bd = disc;
+                    m = 1;
+                } else {
+                    for (int i = 0; i + m <= t2; ++i) lam[i + m] = F.add(lam[i + m], F.neg(F.mul(coef, bp[i])));
+                    ++m;
+                }
+            }
+            bool ok = L <= t2 / 2 && lam[L] != 0;
+            for (int i = L + 1; i <= t2; ++i) ok = ok && lam[i] == 0;
+            if (ok) {
+                for (int a = 0; a < t2; ++a) {                    // Omega = S(x) Lambda(x) mod x^2t
+                    int v = 0;
+                    for (int b2 = 0; b2 <= L && b2 <= a; ++b2) v = F.add(v, F.mul(S[a - b2], lam[b2]));
+                    om[a] = v;
+                }
+                status = 2;
+            }
+        }
+        rs[164] = L;
+        rs[165] = status;
+    }
+    wave_lds_sync();
+    if (rs[165] != 2) return;                                      // no syndrome, or failure: k = 0
+    const int L = rs[164];
+    int roots = 0;
+    bool fail = false;
+    for (int p0 = 0; p0 < nc; p0 += 64) {
+        const int pos = p0 + lane;
+        bool root = false, bad = false;
+        int val = 0;
+        if (pos < nc) {
+            const int xinv = F.alpha(-(nc - 1 - pos));
+            int v = 0;
+            for (int i = L; i >= 0; --i) v = F.add(F.mul(v, xinv), lam[i]);
+            if (v == 0) {
+                root = true;
+                int dv = 0;
+                for (int i = L; i >= 1; --i) dv = F.add(F.mul(dv, xinv), F.mul(i % q, lam[i]));
+                int ov = 0;
+                for (int i = t2 - 1; i >= 0; --i) ov = F.add(F.mul(ov, xinv), om[i]);
+                if (dv == 0) {
+                    bad = true;
+                } else {
+                    val = F.neg(F.mul(ov, F.inv(dv)));
+                    bad = val >= q;
+                }
+            }
+        }
+        roots += __popcll(__ballot(root));
+        fail = fail || __ballot(bad) != 0ull;
+        if (root && !bad && pos >= nc - n) kb8[pos - (nc - n)] = (int8_t)val;
+    }
+    wave_lds_sync();
+    if (roots != L || fail) {
+        uint32_t* kk = reinterpret_cast<uint32_t*>(kb8);
+        if (lane < (d.n_pad >> 2)) kk[lane] = 0;
+        wave_lds_sync();
+    }
+}
+
 template <int NR>
 struct DetectWaveLds {
     __align__(16) uint32_t kws[QSFT_MAX_N / 4];
+    int8_t syn[kRsMaxSyn];                                         // coded source: syndrome digits
+    int32_t rs[kRsScratch];                                        // coded source: decoder state
     __align__(16) PendingSlot pslot[2];                            // first queued ball, per column slot
     __align__(16) double2 colw[2][64 * NR];                        // current / next column
     uint64_t sb_h[kBinsPerWave];
@@ -723,12 +869,16 @@ __device__ __forceinline__ void detect_bins(const qsft_peel_desc& d, int it, int
             if (lane < nw) kk[lane] = 0;
             wave_lds_sync();
 
-            // --- k: one coordinate per lane
-            for (int i0 = 1; i0 <= n; i0 += 64) {
+            // --- k: one coordinate per lane (coded source: one syndrome digit
+            // per lane, decoded into k below)
+            const bool coded = (d.method & QSFT_DETECT_CODED) != 0;
+            const int nch = coded ? d.p1 - 1 : n;
+            int8_t* dig = coded ? W.syn : kb8;
+            for (int i0 = 1; i0 <= nch; i0 += 64) {
                 const int i = i0 + lane;
-                if (i <= n) {
+                if (i <= nch) {
                     uint32_t digit;
-                    if (d.method == QSFT_DETECT_NSO1) {
+                    if (d.method & QSFT_DETECT_NSO1) {
                         const int p1 = d.p1, R = P / p1;
                         double zr = 0.0, zi = 0.0;
                         for (int r = 0; r < R; ++r) {
@@ -744,10 +894,11 @@ __device__ __forceinline__ void detect_bins(const qsft_peel_desc& d, int it, int
                     } else {
                         digit = noiseless_digit(cw[0], cw[i], q);
                     }
-                    kb8[i - 1] = (int8_t)digit;
+                    dig[i - 1] = (int8_t)digit;
                 }
             }
             wave_lds_sync();
+            if (coded) rs_decode_wave(d, W.syn, kb8, W.rs, lane);
 
             // --- signature, rho, residual (rows across lanes); the column is
             // re-read from LDS so its registers are free during the k digits
@@ -1426,10 +1577,12 @@ extern "C" int qsft_peel_bind(qsft_peel_desc* h, int q, int n, int b, int C, int
                               double* U, void* ws, size_t ws_bytes, int n_pad) {
     if (!h || !ws || q < 2 || q > QSFT_MAX_Q || n < 1 || n > QSFT_MAX_N || b < 1 || b > QSFT_MAX_B || C < 1 ||
         C > QSFT_MAX_C || P < 1 || P > 64 * kRowsPerLane || p1 < 2 || P % p1 || max_iter < 1 || max_iter > 15 ||
-        n_pad % 64 || n_pad < n || n_pad > QSFT_MAX_N ||
-        (method != QSFT_DETECT_NSO1 && method != QSFT_DETECT_NOISELESS) ||
-        (method == QSFT_DETECT_NOISELESS && P != n + 1) || (method == QSFT_DETECT_NSO1 && p1 != n + 1)) {
-        qsft_set_error("qsft_peel_bind: bad shape (P <= 256, p1 = n + 1, noiseless needs P = n + 1)");
+        n_pad % 64 || n_pad < n || n_pad > QSFT_MAX_N || (method & ~(QSFT_DETECT_NSO1 | QSFT_DETECT_CODED)) ||
+        ((method & QSFT_DETECT_NSO1) == QSFT_DETECT_NOISELESS && P != p1) ||
+        (!(method & QSFT_DETECT_CODED) && p1 != n + 1) ||
+        ((method & QSFT_DETECT_CODED) && (p1 - 1 > kRsMaxSyn || (p1 - 1) % 2))) {
+        qsft_set_error("qsft_peel_bind: bad shape (P <= 256, p1 = n + 1 or, coded, 2 t s + 1 <= 129; "
+                       "noiseless needs P = p1)");
         return QSFT_ERR_ARG;
     }
     const long long Qll = ipow(q, b);
@@ -1476,7 +1629,28 @@ extern "C" int qsft_peel_bind(qsft_peel_desc* h, int q, int n, int b, int C, int
     d.noise_sigma = 0.0;
     d.flags = 0;
     d.energy = nullptr;
+    d.rs_tab = nullptr;
+    d.rs_t = d.rs_s = d.rs_nc = 0;
     *h = d;
+    return QSFT_OK;
+}
+
+extern "C" int qsft_peel_set_coded(qsft_peel_desc* h, int t, int s, const int16_t* tables) {
+    if (!h || !tables || !(h->method & QSFT_DETECT_CODED) || t < 1 || s < 1 || h->p1 - 1 != 2 * t * s ||
+        2 * t > kRsMaxT2) {
+        qsft_set_error("qsft_peel_set_coded: bad arguments (coded method, p1 - 1 = 2 t s, 2 t <= 32)");
+        return QSFT_ERR_ARG;
+    }
+    long long order = 1;
+    for (int i = 0; i < s; ++i) order *= h->q;
+    if (order > 32768 || order - 1 < (long long)h->n) {
+        qsft_set_error("qsft_peel_set_coded: q^s out of range (n <= q^s - 1 <= 32767)");
+        return QSFT_ERR_ARG;
+    }
+    h->rs_tab = tables;
+    h->rs_t = t;
+    h->rs_s = s;
+    h->rs_nc = (int)(order - 1);
     return QSFT_OK;
 }
 

@@ -489,7 +489,9 @@ class PeelBatch:
     bin-major [C][Q][P]), optional shared_d (all groups use the delay rows of
     group 0; compared on the device when absent), unit_d (+1 / -1: delay rows
     are block offsets + s e_i, 0: arbitrary rows; detected when absent),
-    energy (float64 [C][Q] first-pass column energies written by Assemble into
+    coded (dict t, s, tables: the Reed-Solomon code of coded source delays,
+    reed_solomon.ReedSolomon.device_tables() on the device; method carries
+    QSFT_DETECT_CODED), energy (float64 [C][Q] first-pass column energies written by Assemble into
     the same U) and sparse (dict x = the assembler's FP32 samples [C][P][Q],
     sigma = its Philox sigma or 0): U then holds only the columns the
     assembler wrote (see sparse_targets).
@@ -521,6 +523,10 @@ class PeelBatch:
                                        float(it["cutoff"]), 0, _ptr(it["Mt"]), _ptr(it["D"]), _ptr(it["U"]),
                                        _ptr(ws), nbytes, n_pad), "qsft_peel_bind")
             d.flags = peel_flags(it)
+            cd = it.get("coded")
+            if cd is not None:
+                L.check(lib.qsft_peel_set_coded(ctypes.byref(d), int(cd["t"]), int(cd["s"]), _ptr(cd["tables"])),
+                        "qsft_peel_set_coded")
             if it.get("energy") is not None:
                 d.energy = _ptr(it["energy"])
             sp = it.get("sparse")

@@ -19,6 +19,7 @@ from . import _lib as L
 from . import device as dv
 from . import kernels
 from .input_signal_subsampled import SubsampledSignal
+from .reed_solomon import ReedSolomon
 from .utils import calc_hamming_weight, sort_qary_vecs
 
 MAX_ITER = 15
@@ -35,15 +36,28 @@ class QSFT:
         self.host_loop = bool(kwargs.get("host_loop", False))
 
     def _method(self):
-        if self.reconstruct_method_source not in (None, "identity"):
-            raise NotImplementedError(f"reconstruct_method_source={self.reconstruct_method_source!r} "
-                                      "(only 'identity' is implemented)")
+        src = self.reconstruct_method_source
+        if src not in (None, "identity", "coded"):
+            raise NotImplementedError(f"reconstruct_method_source={src!r} (only 'identity' and 'coded')")
         ch = self.reconstruct_method_channel
         if ch == "nso":
-            return L.QSFT_DETECT_NSO1
-        if ch == "identity":
-            return L.QSFT_DETECT_NOISELESS
-        raise NotImplementedError(f"reconstruct_method_channel={ch!r} (only 'nso' and 'identity')")
+            m = L.QSFT_DETECT_NSO1
+        elif ch == "identity":
+            m = L.QSFT_DETECT_NOISELESS
+        else:
+            raise NotImplementedError(f"reconstruct_method_channel={ch!r} (only 'nso' and 'identity')")
+        return m | (L.QSFT_DETECT_CODED if src == "coded" else 0)
+
+    def _code(self, n, q):
+        """The Reed-Solomon code behind source_decoder (a get_reed_solomon_dec
+        decoder, qsft/query.py:231-243) for coded source delays."""
+        dec = self.source_decoder
+        rs = getattr(dec, "__self__", None)
+        if not isinstance(rs, ReedSolomon):
+            raise ValueError("reconstruct_method_source='coded' needs source_decoder=get_reed_solomon_dec(n, t, q)")
+        if rs.ns != n or rs.q != q:
+            raise ValueError(f"source_decoder is for n={rs.ns}, q={rs.q}; the signal has n={n}, q={q}")
+        return rs
 
     def _working_input(self, signal):
         """(Ms, Ds, U device complex128 bin-major [C][Q][P], Ts) for this call."""
@@ -79,8 +93,12 @@ class QSFT:
         Dd = dv.D_tensor(Ds, n_pad, dev)
         torch.cuda.synchronize()
         t0 = time.time()
-        batch = kernels.PeelBatch([dict(q=q, n=n, b=b, C=C, P=P, p1=signal.get_source_parity(), method=method,
-                                        cutoff=cutoff, Mt=Mt, D=Dd, U=U)], max_iter=MAX_ITER)
+        item = dict(q=q, n=n, b=b, C=C, P=P, p1=signal.get_source_parity(), method=method, cutoff=cutoff, Mt=Mt,
+                    D=Dd, U=U)
+        if method & L.QSFT_DETECT_CODED:
+            rs = self._code(n, q)
+            item["coded"] = dict(t=rs.t, s=rs.s, tables=torch.from_numpy(rs.device_tables()).to(dev))
+        batch = kernels.PeelBatch([item], max_iter=MAX_ITER)
         # passes 3.. as one launch (kernels.PeelBatch.run): the same results, no launch per pass
         batch.run(host_loop=self.host_loop, fuse_from=None if self.host_loop else 3)
         k, vals, _, info = batch.results()[0]

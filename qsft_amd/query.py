@@ -8,6 +8,8 @@ source delays (query.py:79-85).
 """
 import numpy as np
 
+from .reed_solomon import ReedSolomon
+
 
 def get_Ms_simple(n, b, q, num_to_get=None):
     """Block identities, filled from the last group to the first (query.py:13-23)."""
@@ -48,7 +50,8 @@ def get_D_random(n, **kwargs):
 
 
 def get_D_source_coded(n, **kwargs):
-    raise NotImplementedError("coded (Reed-Solomon) source delays are not implemented in qsft_amd")
+    """Reed-Solomon parity-check delays, (2 t s + 1) x n (query.py:88-92; see reed_solomon)."""
+    return np.array(ReedSolomon(n, kwargs.get("t"), kwargs.get("q")).get_delay_matrix(), dtype=int)
 
 
 def get_D_nso(n, D_source, **kwargs):
@@ -89,14 +92,9 @@ def get_Ms_and_Ds(n, q, **kwargs):
 
 
 def get_reed_solomon_dec(n, t_max, q):
-    """Stand-in for the Reed-Solomon syndrome decoder (query.py:231-243).
-
-    Coded source delays are not implemented; the returned decoder raises when
-    called, so scripts that build one but use identity delays still run.
-    """
-    if q not in [2, 3, 5, 7, 11, 13, 15, 17, 19, 23, 29]:
-        raise NotImplementedError("q is not a prime number under 30!")
-
-    def decoder(*args, **kwargs):
-        raise NotImplementedError("coded source delays (Reed-Solomon) are not implemented in qsft_amd")
-    return decoder
+    """A Reed-Solomon syndrome decoder for a t_max error-correcting code (query.py:231-243):
+    the bound `syndrome_decode` of the code, as the reference returns; QSFT
+    reads the code from it and decodes on the device."""
+    if q in [2, 3, 5, 7, 11, 13, 15, 17, 19, 23, 29]:
+        return ReedSolomon(n, t_max, q).syndrome_decode
+    raise NotImplementedError("q is not a prime number under 30!")
