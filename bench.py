@@ -391,7 +391,8 @@ def main():
     value = nreq / el
     sat_lat = SV.latency_summary(reqs)
     nm = [E.nmse(r.result.k, r.result.values, sp) for r, sp in zip(reqs, sat_specs)]
-    log(f"served: {value:.1f} transforms/s ({srv.batches} batches)", args.quiet)
+    host_us = {k: 1e6 * v / nreq for k, v in srv.host_s.items()}
+    log(f"served: {value:.1f} transforms/s ({srv.batches} batches); host us/request {host_us}", args.quiet)
     openl = serve_open(open_specs, 0.8 * value, args.batch, args.inflight, budget, rng)
     log(f"open loop at {0.8 * value:.0f}/s: p50 {openl['p50_ms']:.2f} ms p99 {openl['p99_ms']:.2f} ms", args.quiet)
 
@@ -456,6 +457,7 @@ def main():
                        "of": "per request, arrival -> result in host memory, open-loop Poisson arrivals at 80% of "
                              "the saturation rate",
                        "open_loop": openl, "saturation_p50": sat_lat["p50_ms"], "saturation_p99": sat_lat["p99_ms"]},
+        "serving_host_us_per_request": host_us,
         "plain_fp64": plain,
         "speedup_vs_plain": (value / plain["value"]) if plain else None,
         "nmse": {"mean": float(np.mean(nm)), "max": float(np.max(nm)), "n": len(nm),

@@ -105,7 +105,7 @@ typedef struct qsft_eval_mfma_desc {
     int32_t ptb;            /* p-tiles per workgroup: 5..8 for q in {2, 4} with n_pad = 64,  *
                              * 8 otherwise (same for every instance of a launch)           */
     const int8_t* Mt;       /* [C][b][n_pad]                                                */
-    const int8_t* D;        /* [C][P][n_pad], 16-byte aligned                               */
+    const int8_t* D;        /* [C][P][n_pad] ([P][n_pad] when shared_d), 16-byte aligned    */
     const int8_t* K;        /* [S][n_pad], 16-byte aligned                                  */
     const double* a;        /* [S][2] strengths, complex FP64, 16-byte aligned              */
     uint16_t* Bf;           /* workspace: B fragments, qsft_eval_mfma_workspace_bytes(shared_d ? 1 : C, P, S) */
@@ -278,7 +278,7 @@ typedef struct qsft_peel_desc {
     double cutoff;          /* threshold per row: |col|^2 > cutoff * P                      */
     double peel_cap;        /* q^n (as double, saturating): cap on peeled balls            */
     const int8_t* Mt;       /* [C][b][n_pad]                                                */
-    const int8_t* D;        /* [C][P][n_pad]                                                */
+    const int8_t* D;        /* [C][P][n_pad] ([P][n_pad] with QSFT_PEEL_SHARED_D)           */
     double* U;              /* [C][Q][P] complex FP64 (bin-major), modified in place       */
     const double* energy;   /* optional [C][Q] first-pass |col|^2 from qsft_assemble (set   *
                              * after qsft_peel_bind); NULL: the screen reads U             */
@@ -464,6 +464,9 @@ int qsft_batch_input_offset(void* batch, int i, size_t* offsets4);
  * (qsft_batch_delay_sign). */
 int qsft_batch_pack_input(void* batch, int i, const int8_t* M, const int8_t* D, const int8_t* locq,
                           const double* a, void* input_host);
+/* Every request at once (arrays of n_req pointers), on up to `threads` host threads. */
+int qsft_batch_pack_inputs(void* batch, const int8_t* const* M, const int8_t* const* D, const int8_t* const* locq,
+                           const double* const* a, void* input_host, int threads);
 int qsft_batch_delay_sign(void* batch, int i);
 int qsft_batch_bind(void* batch, void* input_dev, void* work_dev, void* plan_host, void* plan_dev, void* out);
 int qsft_batch_launch(void* batch, int upload, const void* input_host, uint64_t seed_offset, void* stream,

@@ -11,4 +11,5 @@ package itself imports nothing so the build step can run before a library
 exists.
 """
 
-__all__ = ["qsft", "input_signal_subsampled", "synthetic_signal", "query", "utils", "engine", "serving"]
+__all__ = ["qsft", "input_signal_subsampled", "synthetic_signal", "query", "utils", "engine", "serving",
+           "reed_solomon"]

@@ -143,6 +143,7 @@ SIGNATURES = {
     "qsft_batch_create": (vp, [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(BatchSizes)]),
     "qsft_batch_input_offset": (ctypes.c_int, [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_size_t)]),
     "qsft_batch_pack_input": (ctypes.c_int, [vp, ctypes.c_int, vp, vp, vp, vp, vp]),
+    "qsft_batch_pack_inputs": (ctypes.c_int, [vp, vp, vp, vp, vp, vp, ctypes.c_int]),
     "qsft_batch_delay_sign": (ctypes.c_int, [vp, ctypes.c_int]),
     "qsft_batch_bind": (ctypes.c_int, [vp, vp, vp, vp, vp, vp]),
     "qsft_batch_launch": (ctypes.c_int, [vp, ctypes.c_int, vp, ctypes.c_uint64, vp, vp]),

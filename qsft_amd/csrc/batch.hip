@@ -16,6 +16,7 @@
 
 #include <map>
 #include <new>
+#include <thread>
 #include <tuple>
 #include <vector>
 
@@ -64,7 +65,7 @@ bool layout_of(const qsft_request& r, int max_iter, ReqLayout* L) {
     l.sparse = l.energy;
     size_t o = 0;
     l.mt = o; o += al256((size_t)r.C * r.b * l.n_pad);
-    l.d = o;  o += al256((size_t)r.C * l.P * l.n_pad);
+    l.d = o;  o += al256((size_t)l.P * l.n_pad);                  // one copy: every group shares the rows
     l.kraw = o; o += al256((size_t)r.n * r.S);                   // locq as given, [n][S]
     l.a = o;  o += al256((size_t)r.S * 16);
     l.in_size = o;
@@ -531,10 +532,42 @@ extern "C" int qsft_batch_pack_input(void* h, int i, const int8_t* M, const int8
         for (int i2 = 0; i2 < n; ++i2)
             for (int t = 0; t < b; ++t) Mt[((size_t)c * b + t) * np + i2] = M[((size_t)c * n + i2) * b + t];
     for (int p = 0; p < P; ++p) memcpy(Dd + (size_t)p * np, D + (size_t)p * n, (size_t)n);
-    for (int c = 1; c < C; ++c) memcpy(Dd + (size_t)c * P * np, Dd, (size_t)P * np);     // every group: same rows
     memcpy(base + l.kraw, locq, (size_t)n * S);                    // transposed on the device
     memcpy(base + l.a, a, (size_t)S * 16);
     B->sign[i] = delay_sign(D, r.q, n, r.R);
+    return QSFT_OK;
+}
+
+// Every request's inputs, packed by up to `threads` host threads (requests
+// are independent: each writes only its own block of the staging arena).
+extern "C" int qsft_batch_pack_inputs(void* h, const int8_t* const* M, const int8_t* const* D,
+                                      const int8_t* const* locq, const double* const* a, void* input_host,
+                                      int threads) {
+    Batch* B = as_batch(h);
+    if (!B || !M || !D || !locq || !a || !input_host || threads < 1) {
+        qsft_set_error("qsft_batch_pack_inputs: bad arguments");
+        return QSFT_ERR_ARG;
+    }
+    const int n = (int)B->req.size();
+    const int nt = threads < n ? threads : n;
+    std::vector<int> rc(nt, QSFT_OK);
+    auto work = [&](int w) {
+        for (int i = w; i < n; i += nt) {
+            const int r = qsft_batch_pack_input(h, i, M[i], D[i], locq[i], a[i], input_host);
+            if (r != QSFT_OK) { rc[w] = r; return; }
+        }
+    };
+    if (nt == 1) {
+        work(0);
+    } else {
+        std::vector<std::thread> pool;
+        pool.reserve(nt - 1);
+        for (int w = 1; w < nt; ++w) pool.emplace_back(work, w);
+        work(0);
+        for (auto& t : pool) t.join();
+    }
+    for (int r : rc)
+        if (r != QSFT_OK) return r;
     return QSFT_OK;
 }
 

@@ -907,7 +907,8 @@ __device__ __forceinline__ void detect_bins(const qsft_peel_desc& d, int it, int
                 const int p = lane + 64 * r;
                 u[r] = (p < P) ? cw[p] : make_double2(0.0, 0.0);
             }
-            const QSFT_GLOBAL int8_t* __restrict__ Dc = gptr(d.D) + (size_t)c * P * n_pad;
+            const QSFT_GLOBAL int8_t* __restrict__ Dc =
+                gptr(d.D) + (size_t)((d.flags & QSFT_PEEL_SHARED_D) ? 0 : c) * P * n_pad;
             double2 w[NR];
             row_phases(d, Dc, kk, tab, mq, lane, w, offbuf, offs_staged);
             double rr = 0.0, ri = 0.0;

@@ -27,6 +27,7 @@ its batch's results are in host memory and handed to it.
 """
 import collections
 import ctypes
+import os
 import time
 from dataclasses import dataclass
 
@@ -37,6 +38,8 @@ from . import _lib as L
 from .engine import TransformSpec
 
 lib = L.lib
+# host threads that pack a batch's request inputs into its staging arena
+PACK_THREADS = max(1, min(8, (os.cpu_count() or 1) // 2))
 
 
 class AdmissionError(ValueError):
@@ -105,12 +108,13 @@ class ServedBatch:
         self.work_dev = torch.empty(max(int(sz.work_bytes), 256), dtype=torch.uint8, device=device)
         self.plan_dev = torch.empty(max(int(sz.plan_bytes), 256), dtype=torch.uint8, device=device)
         base = self.in_host.data_ptr()
-        pack = lib.qsft_batch_pack_input
+        ptrs = [(ctypes.c_void_p * n)() for _ in range(4)]
         for i, sp in enumerate(self.specs):
-            M, D, locq, a = sp.wire()
-            rc = pack(self.h, i, M.ctypes.data, D.ctypes.data, locq.ctypes.data, a.ctypes.data, base)
-            if rc:
-                L.check(rc, "qsft_batch_pack_input")
+            w = sp.wire()
+            for j in range(4):
+                ptrs[j][i] = w[j].ctypes.data
+        L.check(lib.qsft_batch_pack_inputs(self.h, *ptrs, base, PACK_THREADS if n >= 8 else 1),
+                "qsft_batch_pack_inputs")
         L.check(lib.qsft_batch_bind(self.h, self.in_dev.data_ptr(), self.work_dev.data_ptr(),
                                     self.plan_host.data_ptr(), self.plan_dev.data_ptr(), self.out.data_ptr()),
                 "qsft_batch_bind")
@@ -227,6 +231,7 @@ class Server:
         self.free_slots = list(range(self.max_inflight))
         self.completed = []
         self.batches = 0
+        self.host_s = {"submit": 0.0, "build": 0.0, "launch": 0.0, "retire": 0.0}     # host seconds per phase
 
     def tune(self, specs, max_candidates=6):
         """Autotune the sub-DFT schedule of every (q, b) among `specs` not
@@ -241,6 +246,7 @@ class Server:
             K.dft_autotune(q, b, L.QSFT_F32, rows, max_candidates=max_candidates, device=self.device)
 
     def submit(self, spec, t_submit=None):
+        t0 = time.perf_counter()
         rs = request_struct(spec, self.max_iter)
         fp = int(lib.qsft_request_footprint(ctypes.byref(rs), self.max_iter))
         if fp <= 0:
@@ -249,6 +255,7 @@ class Server:
             raise AdmissionError(f"request needs {fp} bytes of HBM, more than the whole budget {self.budget}")
         req = Request(spec=spec, footprint=fp, t_submit=time.perf_counter() if t_submit is None else t_submit,
                       struct=rs)
+        self.host_s["submit"] += time.perf_counter() - t0
         self.waiting.append(req)
         return req
 
@@ -256,6 +263,7 @@ class Server:
         still = []
         for bt, reqs, slot in self.inflight:
             if bt.done():
+                t0 = time.perf_counter()
                 res = bt.results()
                 now = time.perf_counter()
                 for r, x in zip(reqs, res):
@@ -265,6 +273,7 @@ class Server:
                 self.used -= bt.bytes
                 self.free_slots.append(slot)
                 bt.free()
+                self.host_s["retire"] += time.perf_counter() - t0
             else:
                 still.append((bt, reqs, slot))
         self.inflight = still
@@ -281,14 +290,17 @@ class Server:
             if not take:
                 return
             slot = self.free_slots.pop()
+            t0 = time.perf_counter()
             bt = ServedBatch([r.spec for r in take], self.device, stream=self.streams[slot],
                              tail_stream=self.tails[slot], max_iter=self.max_iter, fuse_from=self.fuse_from,
                              structs=[r.struct for r in take])
             now = time.perf_counter()
+            self.host_s["build"] += now - t0
             for r in take:
                 r.t_start = now
                 r.batch_size = len(take)
             bt.launch()
+            self.host_s["launch"] += time.perf_counter() - now
             self.used += bt.bytes
             self.inflight.append((bt, take, slot))
             self.batches += 1
