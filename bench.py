@@ -258,15 +258,17 @@ def resident_engine(args, E, torch):
     flops = 8.0 * samples * S * args.batch                         # B*S complex multiply-adds per launch
     achieved = flops / (gemm_ms / 1e3) / 1e12
     from qsft_amd.kernels import mfma_mode, mfma_ptb
-    nsb, npt = (S + 15) // 16, (R * (n + 1) + 7) // 8
+    nsp, npt = (S + 31) // 32, (R * (n + 1) + 7) // 8
     ptb = mfma_ptb(mfma_mode(q), 1, npt)
     npt_run = ptb * ((npt + ptb - 1) // ptb)
-    mfma_f16 = 2 * 2.0 * C * q ** b * (32 * nsb) * (16 * npt_run) * args.batch
-    roof = {"kernel": "k_eval_mfma (int8 MFMA phase + fp16x2 MFMA complex GEMM, FP32 accumulation)",
+    # q in {2, 4}: v_smfmac_f32_16x16x64_f16, 32 supports x 16 queries x 16 columns per
+    # instruction (one nonzero of each (Re W, Im W) k-row pair), two fp16 limbs of B
+    mfma_f16 = 2 * 2.0 * C * q ** b * (32 * nsp) * (16 * npt_run) * args.batch
+    roof = {"kernel": "k_eval_smfmac_occ2 (int8 MFMA phase + fp16x2 sparse-MFMA complex GEMM, FP32 accumulation)",
             "bound": "mfma", "achieved": achieved, "peak": PEAK_F16_TFLOPS, "unit": "TFLOP/s",
             "frac": achieved / PEAK_F16_TFLOPS, "traffic": measured_traffic(args.batch),
             "algorithmic_flops_per_launch": flops, "launch_ms": gemm_ms,
-            "mfma_f16_executed_tflops": mfma_f16 / (gemm_ms / 1e3) / 1e12}
+            "mfma_f16_nonzero_executed_tflops": mfma_f16 / (gemm_ms / 1e3) / 1e12}
     out = {"value": args.batch * steps / elapsed, "ms_per_step": 1e3 * elapsed / steps,
            "latency_ms": {"p50": 1e3 * float(np.percentile(lat, 50)), "p99": 1e3 * float(np.percentile(lat, 99)),
                           "of": "a batch from launch to rows on the host, two batches in flight"},

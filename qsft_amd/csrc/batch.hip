@@ -233,14 +233,16 @@ int build_plan(Batch& B, bool write) {
     }
 
     // ---- stage (a)+(b): matrix-core evaluation, grouped by kernel variant
-    std::map<std::tuple<int, int, int>, std::vector<int>> eg;
+    std::map<std::tuple<int, int, int, int>, std::vector<int>> eg;
     for (int i = 0; i < nreq; ++i) {
         const qsft_request& r = B.req[i];
         const int nkb = B.lay[i].n_pad / 64, mode = mfma_mode_of(r.q);
-        eg[std::make_tuple(nkb, mode, mfma_ptb_of(mode, nkb, (B.lay[i].P + 7) / 8))].push_back(i);
+        eg[std::make_tuple(nkb, mode, mfma_ptb_of(mode, nkb, (B.lay[i].P + 7) / 8),
+                           (int)qsft::mfma_wide(r.q, r.b, B.lay[i].n_pad))].push_back(i);
     }
     for (auto& kv : eg) {
         const int nkb = std::get<0>(kv.first), mode = std::get<1>(kv.first), ptb = std::get<2>(kv.first);
+        const int wide = std::get<3>(kv.first);
         std::vector<qsft_eval_mfma_desc> descs;
         std::vector<int> prep, mainb;
         for (int i : kv.second) {
@@ -271,7 +273,7 @@ int build_plan(Batch& B, bool write) {
         L.pre_main = W.prefix(mainb, &L.main_blocks);
         if (L.prep_blocks < 0 || L.main_blocks < 0) { qsft_set_error("qsft_batch: launch too large"); return QSFT_ERR_ARG; }
         L.n = (int)descs.size();
-        L.word = (nkb << 4) | (mode << 8) | (ptb << 12);
+        L.word = (nkb << 4) | (mode << 8) | (wide << 11) | (ptb << 12);
         B.mfma.push_back(L);
     }
 

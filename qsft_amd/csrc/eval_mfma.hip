@@ -47,13 +47,13 @@ typedef unsigned int v4u __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 typedef float v2f __attribute__((ext_vector_type(2)));
 typedef _Float16 v8h __attribute__((ext_vector_type(8)));
+typedef _Float16 v16h __attribute__((ext_vector_type(16)));
+typedef unsigned int v8u __attribute__((ext_vector_type(8)));
 
-constexpr int kMfThreads = 256;
-constexpr int kRowsPerBlock = 256;   // base queries per workgroup
+constexpr int kMfThreads = 256;     // workgroup of the generic kernels (4 waves, 256 base queries)
 constexpr int kPtPerBlock = 8;       // most p-tiles (8 complex delay rows each) per workgroup
 constexpr int kLimbs = 2;            // fp16 limbs of B (and of W for q not in {2, 4})
 constexpr int kStage = kPtPerBlock * kLimbs * 64;      // v4u per staged support block
-constexpr int kStagePerThread = kStage / kMfThreads;   // 4
 constexpr int kXcds = 8;
 
 // fp16 bits of f (round to nearest even)
@@ -91,11 +91,23 @@ __device__ __forceinline__ int xcd_logical_block() {
 }
 
 // Prep: B_c[s][p] = 2^bexp a_s w^{<d_cp, k_s>} in fp16-limb B-fragment order.
-// Fragment (c, sb, pt, limb) is 64 lanes x 8 fp16; lane = 16 g + n with
-// n = 2 (p % 8) + {0: Re column, 1: Im column}; slot j < 4 <-> "Re W" row of
-// support s = 16 sb + 4 g + j, slot 4 + j <-> "Im W" row of the same support.
-// One thread fills lanes (16 g + 2 pl, 16 g + 2 pl + 1) of every limb for one
-// (c, sb, pt, g, pl): two adjacent 16-byte lanes per limb.
+// Dense (SP = false), for v_mfma_f32_16x16x32_f16: fragment (c, sb, pt, limb)
+// is 64 lanes x 8 fp16; lane = 16 g + n with n = 2 (p % 8) + {0: Re column,
+// 1: Im column}; slot j < 4 <-> "Re W" row of support s = 16 sb + 4 g + j,
+// slot 4 + j <-> "Im W" row of the same support. One thread fills lanes
+// (16 g + 2 pl, 16 g + 2 pl + 1) of every limb for one (c, sb, pt, g, pl): two
+// adjacent 16-byte lanes per limb.
+// Sparse (SP = true), for v_smfmac_f32_16x16x64_f16 (gfx950 lane layout: B
+// element J of lane 16 g + n is k-row 32 (J / 8) + 8 g + J % 8 of column n):
+// fragment (c, SB, pt, limb) is 64 lanes x 16 fp16 over 32 supports, k-row
+// 2 s' + {0: Re W, 1: Im W} for support s' of the block. Half h = J / 8 of
+// lane 16 g + n holds supports 32 SB + 16 (g % 2) + 8 h + 4 (g / 2) + j,
+// j = 0..3, as (Re W row, Im W row) pairs; that order matches the compressed
+// A fragments built from the E0 tiles in the main kernel. Each half is
+// stored lane-major (half h of lane L at 16-byte unit 64 h + L of the
+// fragment), so a wave reads a half as one contiguous, conflict-free 1 KiB.
+// One thread fills half h of lanes (16 g + 2 pl, 16 g + 2 pl + 1) of every limb.
+template <bool SP>
 __global__ __launch_bounds__(256) void k_eval_mfma_prep(const qsft_eval_mfma_desc* __restrict__ descs,
                                                         const int32_t* __restrict__ pre, int n_inst) {
     __shared__ double2 tab[QSFT_MAX_Q];
@@ -109,8 +121,9 @@ __global__ __launch_bounds__(256) void k_eval_mfma_prep(const qsft_eval_mfma_des
     }
     __syncthreads();
     const int Cb = d.shared_d ? 1 : d.C;
+    const int nsbf = SP ? (nsb + 1) / 2 : nsb;              // fragment blocks (32 or 16 supports)
     const int64_t idx = (int64_t)(blockIdx.x - pre[inst]) * blockDim.x + threadIdx.x;
-    const int64_t per_c = (int64_t)nsb * npt * 32;          // (sb, pt, g, pl)
+    const int64_t per_c = (int64_t)nsbf * npt * (SP ? 64 : 32);   // (sb, pt, [h,] g, pl)
     if (idx >= per_c * Cb) return;
     const int c = (int)(idx / per_c);
     int64_t rem = idx - c * per_c;
@@ -118,9 +131,16 @@ __global__ __launch_bounds__(256) void k_eval_mfma_prep(const qsft_eval_mfma_des
     rem >>= 3;
     const int g = (int)(rem & 3);
     rem >>= 2;
+    int h = 0;
+    if constexpr (SP) {
+        h = (int)(rem & 1);
+        rem >>= 1;
+    }
     const int pt = (int)(rem % npt);
     const int sb = (int)(rem / npt);
     const int p = 8 * pt + pl;
+    // first of the thread's four supports
+    const int s0 = SP ? 32 * sb + 16 * (g & 1) + 8 * h + 4 * (g >> 1) : 16 * sb + 4 * g;
     const ModQ mq((uint32_t)q);
     const int bexp = d.bexp;
     // <d_p, k_s> for the thread's four supports: 16-byte loads of the delay row
@@ -130,7 +150,7 @@ __global__ __launch_bounds__(256) void k_eval_mfma_prep(const qsft_eval_mfma_des
     const int4* __restrict__ kr[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j)
-        kr[j] = reinterpret_cast<const int4*>(d.K + (size_t)min(16 * sb + 4 * g + j, d.S - 1) * n_pad);
+        kr[j] = reinterpret_cast<const int4*>(d.K + (size_t)min(s0 + j, d.S - 1) * n_pad);
     int acc[4] = {0, 0, 0, 0};
     for (int w = 0; w < (n_pad >> 4); ++w) {
         const int4 dv = dr[w];
@@ -146,7 +166,7 @@ __global__ __launch_bounds__(256) void k_eval_mfma_prep(const qsft_eval_mfma_des
     uint32_t lr[4][kLimbs], li[4][kLimbs];
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-        const int s = 16 * sb + 4 * g + j;
+        const int s = s0 + j;
         double br = 0.0, bi = 0.0;
         if (s < d.S && p < d.P) {
             const double2 wv = tab[mq((uint32_t)acc[j])];
@@ -158,6 +178,20 @@ __global__ __launch_bounds__(256) void k_eval_mfma_prep(const qsft_eval_mfma_des
         split2(bi, li[j]);
     }
     v4u* __restrict__ out = reinterpret_cast<v4u*>(d.Bf);
+    if constexpr (SP) {
+#pragma unroll
+        for (int limb = 0; limb < kLimbs; ++limb) {
+            const size_t frag = ((((size_t)c * nsbf + sb) * npt + pt) * kLimbs + limb) * 128;
+            // Re x column: (Br, -Bi) per support; Im x column: (Bi, Br)
+            const v4u re = v4u{lr[0][limb] | ((li[0][limb] ^ 0x8000u) << 16), lr[1][limb] | ((li[1][limb] ^ 0x8000u) << 16),
+                               lr[2][limb] | ((li[2][limb] ^ 0x8000u) << 16), lr[3][limb] | ((li[3][limb] ^ 0x8000u) << 16)};
+            const v4u im = v4u{li[0][limb] | (lr[0][limb] << 16), li[1][limb] | (lr[1][limb] << 16),
+                               li[2][limb] | (lr[2][limb] << 16), li[3][limb] | (lr[3][limb] << 16)};
+            out[frag + 64 * h + 16 * g + 2 * pl] = re;
+            out[frag + 64 * h + 16 * g + 2 * pl + 1] = im;
+        }
+        return;
+    }
 #pragma unroll
     for (int limb = 0; limb < kLimbs; ++limb) {
         const size_t frag = ((((size_t)c * nsb + sb) * npt + pt) * kLimbs + limb) * 64;
@@ -193,15 +227,32 @@ __device__ __forceinline__ void ring_barrier() {
 // MODE 0: q in {2, 4} (w^e exact in fp16, q a power of two)
 // MODE 1: other powers of two (2-limb W)   MODE 2: any other q (2-limb W)
 //
+// SP (MODE 0 only): the sparse form. For q in {2, 4} w^E0 is one of 1, i,
+// -1, -i, so of the two k-rows (Re W, Im W) of a support exactly one is
+// nonzero (+-1): the 2:4 structured sparsity of v_smfmac_f32_16x16x64_f16,
+// which takes a compressed A (8 fp16 per lane) plus 2-bit row indices and
+// covers 32 supports per instruction at the issue cost of the dense 16x16x32
+// one. A pair of ring slots then holds one 32-support block (16 of its fp16
+// B fragments per slot); the two E0 tiles of the block come from the two
+// digit slots, and every product is the same exact fp16 product as in the
+// dense form. gfx950 lane layout of the sparse instruction: A lane 16 g + m
+// holds row m, k-rows 16 g + 4 (c / 2) + pos_c for its elements c = 0..7,
+// pos_c = bits 2c..2c+1 of the index word (bits 0..15 with abid = 0); B and D
+// as in the prep kernel and the dense instruction.
+//
 // Support blocks stream through four LDS slots, two pairs, filled by
 // global->LDS loads (16 B per lane, lane L at slot base + 16 L): while the
 // MFMAs of one pair run, the other pair's loads are in flight, and no
 // registers hold staged data. The slots are separate arrays and the loop is
 // unrolled by four, so every LDS access names its slot. One wait and one
 // barrier per pair of support blocks.
-template <int MODE, int NKB, int NPT>
+template <int MODE, int NKB, int NPT, bool SP, int NW>
 __device__ __forceinline__ void eval_mfma_body(const qsft_eval_mfma_desc* __restrict__ descs,
                                                const int32_t* __restrict__ pre, int n_inst) {
+    static_assert(!SP || MODE == 0, "the sparse form needs w^e in {1, i, -1, -i}");
+    constexpr int kThreads = 64 * NW;                // NW waves of 64 base queries each
+    constexpr int kRowsPerBlock = kThreads;          // base queries per workgroup
+    constexpr int kStagePerThread = kStage / kThreads;   // B loads per lane per slot
     constexpr int WL = (MODE == 0) ? 1 : kLimbs;
     __shared__ __align__(16) v4u bs0[kStage], bs1[kStage], bs2[kStage], bs3[kStage];   // 4 x 16 KiB of B fragments
     __shared__ __align__(16) v4i ks0[NKB * 64], ks1[NKB * 64], ks2[NKB * 64], ks3[NKB * 64];   // support digits
@@ -211,7 +262,9 @@ __device__ __forceinline__ void eval_mfma_body(const qsft_eval_mfma_desc* __rest
     const int inst = find_inst(pre, n_inst, lblock);
     const qsft_eval_mfma_desc& d = descs[inst];
     if (d.ptb != NPT) return;                                      // launched for another split (host error)
-    const int q = d.q, b = d.b, n_pad = d.n_pad, nsb = d.nsb, npt = d.npt, P = d.P, S = d.S;
+    if (mfma_wide(d.q, d.b, d.n_pad) != (NW == 8)) return;         // ... or another workgroup size
+    const int q = d.q, b = d.b, n_pad = d.n_pad, npt = d.npt, P = d.P, S = d.S;
+    const int nsb = SP ? 2 * ((d.nsb + 1) / 2) : d.nsb;          // 16-support blocks (even when sparse)
     const int Q = ipow(q, b);
     const int nlb = (Q + kRowsPerBlock - 1) / kRowsPerBlock;
     const int ncg = (npt + NPT - 1) / NPT;
@@ -223,7 +276,7 @@ __device__ __forceinline__ void eval_mfma_body(const qsft_eval_mfma_desc* __rest
     const int cb = d.shared_d ? 0 : c;                            // group whose B fragments apply
     const int pt0 = cg * NPT;
     const int npt_here = min(NPT, npt - pt0);
-    const int nstage = npt_here * kLimbs * 64;
+    const int nstage = npt_here * kLimbs * (SP ? 128 : 64);        // 16-byte units of one (pair of) slot(s)
 
     for (int r = threadIdx.x; r < q; r += blockDim.x) {
         double s, cc;
@@ -267,6 +320,13 @@ __device__ __forceinline__ void eval_mfma_body(const qsft_eval_mfma_desc* __rest
             }
 #pragma unroll
             for (int jj = 0; jj < 16; ++jj) acc[jj] = (int)mq((uint32_t)acc[jj]);
+            if constexpr (SP) {
+                // digits times 64 (q = 4) or 128 (q = 2), mod 256: the low byte of
+                // E0 then holds E0 mod 4 in bits 6..7 (or E0 mod 2 in bit 7), bits
+                // 0..5 zero -- the sign / Im flags the sparse W fragments need
+#pragma unroll
+                for (int jj = 0; jj < 16; ++jj) acc[jj] <<= (q == 2) ? 7 : 6;
+            }
             bq[t][kb] = pack16b(acc);
         }
     }
@@ -288,8 +348,8 @@ __device__ __forceinline__ void eval_mfma_body(const qsft_eval_mfma_desc* __rest
         const v4u* bbase = Bg + (((size_t)cb * nsb + sb) * npt + pt0) * kLimbs * 64;
 #pragma unroll
         for (int i = 0; i < kStagePerThread; ++i) {
-            const uint32_t e = (uint32_t)min(tid + i * kMfThreads, nstage - 1);
-            dma16(bbase, e * 16u, __builtin_amdgcn_readfirstlane(lds_addr(bdst + i * kMfThreads + wave * 64)));
+            const uint32_t e = (uint32_t)min(tid + i * kThreads, nstage - 1);
+            dma16(bbase, e * 16u, __builtin_amdgcn_readfirstlane(lds_addr(bdst + i * kThreads + wave * 64)));
         }
         if (wave < NKB) {
             const int s = min(16 * sb + r16, S - 1);
@@ -405,6 +465,177 @@ __device__ __forceinline__ void eval_mfma_body(const qsft_eval_mfma_desc* __rest
             }
         }
     };
+    // Sparse W fragments of subtile t from its two E0 tiles (elements c = 4 u + v
+    // from tile u, lane v): support c of the lane's group is +-1 in its Re W
+    // (pos 2 (c % 2)) or Im W (pos 2 (c % 2) + 1) k-row. With the scaled query
+    // digits the low byte of each E0 is f << 6, f = E0 mod 4 (q = 4; q = 2:
+    // f = 2 (E0 mod 2)): bit 7 is the sign of w^f, bit 6 its Im flag. VALU work
+    // competes with the MFMAs for issue slots, so this is 16 operations per
+    // subtile: two byte permutes + one and-or per pair of fp16 values, and the
+    // index from two gathers, two masks and two byte dot products.
+    const uint32_t sp_one = 0x3C003C00u;                         // two fp16 1.0
+    auto sp_w = [&](const v4i ea, const v4i eb, v4u& wa, uint32_t& wi) {
+        uint32_t pk[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const v4i& e = (k < 2) ? ea : eb;
+            const int h = k & 1;
+            // bytes [0, E0 byte of element 2k, 0, E0 byte of element 2k + 1]
+            pk[k] = __builtin_amdgcn_perm((uint32_t)e[2 * h + 1], (uint32_t)e[2 * h], 0x040C000Cu);
+            // +-1.0 in fp16: (pk & 0x80008000) | 0x3C003C00 as one VOP3 (one
+            // constant in an SGPR, the other in a VGPR: one constant-bus read;
+            // the compiler would split it into two VOP2s with literals)
+            __asm__("v_and_or_b32 %0, %1, %2, %3" : "=v"(wa[k]) : "v"(pk[k]), "s"(0x80008000u), "v"(sp_one));
+        }
+        // E0 bytes of elements 0..3 and 4..7; the Im flags (bit 6 of each byte)
+        // weighted by 4^c give im_c at bit 2c (times 64); index bits 2c..2c+1
+        // = 2 (c % 2) + im_c, i.e. 0x8888 plus im_c at bit 2c
+        const uint32_t q0 = __builtin_amdgcn_perm(pk[1], pk[0], 0x07050301u);
+        const uint32_t q1 = __builtin_amdgcn_perm(pk[3], pk[2], 0x07050301u);
+        const uint32_t d0 = __builtin_amdgcn_udot4(q0 & 0x40404040u, 0x40100401u, 0x8888u << 6, false);
+        const uint32_t d1 = __builtin_amdgcn_udot4(q1 & 0x40404040u, 0x40100401u, 0u, false);
+        wi = (d0 >> 6) | (d1 << 2);
+    };
+    // Sparse ring: a pair of B slots holds one 32-support block m (its 16
+    // fragments, f = 2 pt + limb in slot f / 8); the digits travel one block
+    // ahead: the loads of block m's B fragments carry block m + 1's digits,
+    // so that block m + 1's W fragments are built while block m's MFMAs run.
+    const int nsp = nsb / 2;                                       // 32-support blocks
+    // wave-uniform parts of the load addresses, computed once (each VALU
+    // operation in the loop costs MFMA issue slots)
+    const uint32_t wave_lds = (uint32_t)__builtin_amdgcn_readfirstlane(wave) * 1024u;   // 64 lanes x 16 B
+    const uint64_t kd_base = uniform_base(Kd);
+    uint32_t boff[2][kStagePerThread];                             // per-lane B offsets (bytes)
+#pragma unroll
+    for (int u = 0; u < 2; ++u)
+#pragma unroll
+        for (int i = 0; i < kStagePerThread; ++i) boff[u][i] = 16u * (uint32_t)min(u * kStage + tid + i * kThreads, nstage - 1);
+    auto issue_sp = [&](int m, v4u* d0, v4u* d1) {                 // B fragments of block m
+        const uint64_t gb = uniform_base(Bg + (((size_t)cb * nsp + m) * npt + pt0) * kLimbs * 128);
+        const uint32_t l0 = lds_addr(d0) + wave_lds, l1 = lds_addr(d1) + wave_lds;
+#pragma unroll
+        for (int u = 0; u < 2; ++u)
+#pragma unroll
+            for (int i = 0; i < kStagePerThread; ++i) dma16_u(gb, boff[u][i], (u ? l1 : l0) + 16u * i * kThreads);
+    };
+    auto issue_kd = [&](int m, v4i* d0, v4i* d1) {                 // digits of block m (waves < NKB)
+        if (wave < NKB) {
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                const int s = min(32 * m + 16 * u + r16, S - 1);
+                dma16_u(kd_base, (uint32_t)(s * n_pad + 64 * wave + 16 * g), lds_addr(u ? d1 : d0) + wave_lds);
+            }
+        }
+    };
+    // all W fragments of a block at once (the first block)
+    auto sp_wall = [&](const v4i* k0, const v4i* k1, v4u (&wa)[4], uint32_t (&wi)[4]) {
+        v4i kf0[NKB], kf1[NKB];
+#pragma unroll
+        for (int kb = 0; kb < NKB; ++kb) {
+            kf0[kb] = k0[kb * 64 + lane];
+            kf1[kb] = k1[kb * 64 + lane];
+        }
+        v4i ea[4], eb[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            ea[t] = e0_tile(kf0, t);
+            eb[t] = e0_tile(kf1, t);
+        }
+#pragma unroll
+        for (int t = 0; t < 4; ++t) sp_w(ea[t], eb[t], wa[t], wi[t]);
+    };
+    // The GEMM of one block with W fragments (wa, wi). With NEXT, the next
+    // block's W fragments (digits in kn0, kn1) are built in the first six of
+    // the 2 NPT fragment steps: digit reads after step 0, subtile t's two int8
+    // products after step t + 1 and its byte work after step t + 2, so that
+    // the VALU work runs while this wave's MFMAs execute; scheduling barriers
+    // keep that order.
+    auto sp_gemm = [&](const v4u* b0, const v4u* b1, const v4u (&wa)[4], const uint32_t (&wi)[4], const v4i* kn0,
+                       const v4i* kn1, v4u (&wn)[4], uint32_t (&win)[4], auto next) {
+        constexpr bool NEXT = decltype(next)::value;
+        auto frag = [&](int f, v4u& lo, v4u& hi) {
+            const v4u* base = (f < 8) ? b0 : b1;
+            const int off = (f & 7) * 128 + lane;
+            lo = base[off];
+            hi = base[off + 64];
+        };
+        v4i kf0[NKB], kf1[NKB];
+        v4i ea, eb;
+        v4u nlo, nhi;
+        frag(0, nlo, nhi);
+#pragma unroll
+        for (int st = 0; st < kLimbs * NPT; ++st) {
+            const int bl = st / NPT, pt = st % NPT;
+            const v16h bf = __builtin_bit_cast(v16h, v8u{nlo[0], nlo[1], nlo[2], nlo[3], nhi[0], nhi[1], nhi[2], nhi[3]});
+            if (st + 1 < kLimbs * NPT) frag(2 * ((st + 1) % NPT) + (st + 1) / NPT, nlo, nhi);
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+                acc[t][pt] = __builtin_amdgcn_smfmac_f32_16x16x64_f16(__builtin_bit_cast(v8h, wa[t]), bf, acc[t][pt],
+                                                                     (int)wi[t], 0, 0);
+            if constexpr (NEXT) {
+                if (st == 0) {
+#pragma unroll
+                    for (int kb = 0; kb < NKB; ++kb) {
+                        kf0[kb] = kn0[kb * 64 + lane];
+                        kf1[kb] = kn1[kb * 64 + lane];
+                    }
+                }
+                if (st >= 2 && st <= 5) {
+                    const int t = st - 2;
+                    sp_w(ea, eb, wn[t], win[t]);
+                    __asm__ volatile("" : "+v"(wn[t]), "+v"(win[t]));   // materialise here
+                }
+                if (st >= 1 && st <= 4) {
+                    ea = e0_tile(kf0, st - 1);
+                    eb = e0_tile(kf1, st - 1);
+                }
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    };
+
+    if constexpr (SP) {
+        // prologue: digits of blocks 0 and 1, B of block 0; W of block 0; then
+        // B of block 1 with the digits of block 2 (into block 0's digit slots)
+        issue_kd(0, ks0, ks1);
+        issue_sp(0, bs0, bs1);
+        if (nsp > 1) issue_kd(1, ks2, ks3);
+        wait_vm<0>();
+        ring_barrier();
+        v4u wa0[4], wa1[4];
+        uint32_t wi0[4], wi1[4];
+        sp_wall(ks0, ks1, wa0, wi0);
+        wait_lgkm0();
+        ring_barrier();                              // every wave is done with block 0's digit slots
+        if (nsp > 1) {
+            issue_sp(1, bs2, bs3);
+            if (nsp > 2) issue_kd(2, ks0, ks1);
+        }
+        // block m in B pair m % 2 with W (wa, wi) of parity m % 2; block m + 1's
+        // digits in digit pair (m + 1) % 2
+        for (int m = 0; m < nsp; m += 2) {
+            // the last block also builds "next" W fragments from stale digit
+            // slots (no load is in flight into them); they are never used, and
+            // one copy of the GEMM code keeps the register allocation spill-free
+            sp_gemm(bs0, bs1, wa0, wi0, ks2, ks3, wa1, wi1, std::true_type{});
+            if (m + 1 >= nsp) break;
+            wait_vm<0>();                            // B of block m + 1, digits of block m + 2
+            ring_barrier();                          // ... for every wave; B pair 0, digit pair 1 free
+            if (m + 2 < nsp) {
+                issue_sp(m + 2, bs0, bs1);
+                if (m + 3 < nsp) issue_kd(m + 3, ks2, ks3);
+            }
+            sp_gemm(bs2, bs3, wa1, wi1, ks0, ks1, wa0, wi0, std::true_type{});
+            if (m + 2 >= nsp) break;
+            wait_vm<0>();                            // B of block m + 2, digits of block m + 3
+            ring_barrier();                          // ... for every wave; B pair 1, digit pair 0 free
+            if (m + 3 < nsp) {
+                issue_sp(m + 3, bs2, bs3);
+                if (m + 4 < nsp) issue_kd(m + 4, ks0, ks1);
+            }
+        }
+    } else {
     // The four slots form two pairs; a barrier closes each pair of support
     // blocks. While pair A is computed, pair B's loads are in flight; after
     // the barrier (every wave is done with pair A) pair A is refilled with the
@@ -445,6 +676,7 @@ __device__ __forceinline__ void eval_mfma_body(const qsft_eval_mfma_desc* __rest
         ring_barrier();                              // ... for every wave; pair B is free
         if (sb + 6 < nsb) issue(sb + 6, bs2, ks2);
         if (sb + 7 < nsb) issue(sb + 7, bs3, ks3);
+    }
     }
 
     // store: lane holds column n = r16 -> delay row p = 8 (pt0 + pt) + n / 2, Re (n even) / Im (n odd),
@@ -488,18 +720,33 @@ __device__ __forceinline__ void eval_mfma_body(const qsft_eval_mfma_desc* __rest
 template <int MODE, int NKB>
 __global__ __launch_bounds__(kMfThreads) void k_eval_mfma(const qsft_eval_mfma_desc* __restrict__ descs,
                                                           const int32_t* __restrict__ pre, int n_inst) {
-    eval_mfma_body<MODE, NKB, kPtPerBlock>(descs, pre, n_inst);
+    eval_mfma_body<MODE, NKB, kPtPerBlock, false, 4>(descs, pre, n_inst);
 }
 
 // Two waves per SIMD: caps the body at 256 VGPR+AGPR. Only the one-limb phase
-// (q in {2, 4}) with one k-block (n <= 64) fits that without scratch; it then
-// hides the LDS and global latency of one block behind the MFMAs of the other.
-// NPT (5..8) p-tiles per workgroup, chosen so that the tiles of the delay rows
-// split evenly over the workgroups (13 tiles: 2 x 7, not 2 x 8).
-template <int MODE, int NKB, int NPT>
-__global__ __launch_bounds__(kMfThreads) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_eval_mfma_occ2(
+// (q in {2, 4}) with one k-block (n <= 64) fits that without scratch; each
+// SIMD then hides the LDS and global latency of one wave behind the MFMAs of
+// the other. NW = 8 waves (512 base queries, for q^b >= 512) per workgroup,
+// one workgroup per CU: every staged B fragment feeds twice the queries of a
+// four-wave workgroup, which halves the L2 -> LDS traffic; NW = 4 for smaller
+// q^b. NPT (5..8) p-tiles per workgroup, chosen so that the tiles of the delay
+// rows split evenly over the workgroups (13 tiles: 2 x 7, not 2 x 8).
+template <int MODE, int NKB, int NPT, int NW>
+__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_eval_mfma_occ2(
     const qsft_eval_mfma_desc* __restrict__ descs, const int32_t* __restrict__ pre, int n_inst) {
-    eval_mfma_body<MODE, NKB, NPT>(descs, pre, n_inst);
+    eval_mfma_body<MODE, NKB, NPT, false, NW>(descs, pre, n_inst);
+}
+
+// The sparse form of MODE 0, same two entries.
+template <int NKB>
+__global__ __launch_bounds__(kMfThreads) void k_eval_smfmac(const qsft_eval_mfma_desc* __restrict__ descs,
+                                                            const int32_t* __restrict__ pre, int n_inst) {
+    eval_mfma_body<0, NKB, kPtPerBlock, true, 4>(descs, pre, n_inst);
+}
+template <int NKB, int NPT, int NW>
+__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_eval_smfmac_occ2(
+    const qsft_eval_mfma_desc* __restrict__ descs, const int32_t* __restrict__ pre, int n_inst) {
+    eval_mfma_body<0, NKB, NPT, true, NW>(descs, pre, n_inst);
 }
 
 }  // namespace qsft
@@ -517,17 +764,27 @@ static bool mfma_desc_ok(const qsft_eval_mfma_desc* h) {
            (((uintptr_t)h->D | (uintptr_t)h->K | (uintptr_t)h->a) & 15) == 0;
 }
 
+// Sized for the sparse layout (support blocks rounded up to a multiple of 32),
+// which also holds the dense one.
 extern "C" size_t qsft_eval_mfma_workspace_bytes(int C, int P, int S) {
     if (C < 1 || P < 1 || S < 0) return 0;
-    return (size_t)C * ((S + 15) / 16) * ((P + 7) / 8) * kLimbs * 1024;
+    return (size_t)C * (2 * ((S + 31) / 32)) * ((P + 7) / 8) * kLimbs * 1024;
+}
+
+// base queries per workgroup of the main kernel
+static int mfma_rows_per_block(const qsft_eval_mfma_desc* h) {
+    return mfma_wide(h->q, h->b, h->n_pad) ? 2 * kMfThreads : kMfThreads;
 }
 
 extern "C" int qsft_eval_mfma_blocks(const qsft_eval_mfma_desc* h, int* prep_blocks, int* main_blocks) {
     if (!mfma_desc_ok(h) || !prep_blocks || !main_blocks) return -1;
     const long long Q = ipow(h->q, h->b);
     const long long Cb = h->shared_d ? 1 : h->C;
-    const long long prep = (Cb * h->nsb * h->npt * 32 + 255) / 256;
-    const long long mainb = (long long)h->C * ((Q + kRowsPerBlock - 1) / kRowsPerBlock) *
+    // threads for the sparse layout (16-support blocks rounded up to even);
+    // the dense prep returns early on the extra ones
+    const long long prep = (Cb * (2 * ((h->nsb + 1) / 2)) * h->npt * 32 + 255) / 256;
+    const long long rows = mfma_rows_per_block(h);
+    const long long mainb = (long long)h->C * ((Q + rows - 1) / rows) *
                             ((h->npt + h->ptb - 1) / h->ptb);
     if (prep > 0x7FFFFFFF || mainb > 0x7FFFFFFF) return -1;
     *prep_blocks = (int)prep;
@@ -535,18 +792,50 @@ extern "C" int qsft_eval_mfma_blocks(const qsft_eval_mfma_desc* h, int* prep_blo
     return 0;
 }
 
+template <int NW>
+static void launch_smfmac_occ2(int ptb, const qsft_eval_mfma_desc* d, const int32_t* pre, int n, int blocks,
+                               hipStream_t st) {
+    switch (ptb) {
+        case 5: hipLaunchKernelGGL((k_eval_smfmac_occ2<1, 5, NW>), dim3(blocks), dim3(64 * NW), 0, st, d, pre, n); break;
+        case 6: hipLaunchKernelGGL((k_eval_smfmac_occ2<1, 6, NW>), dim3(blocks), dim3(64 * NW), 0, st, d, pre, n); break;
+        case 7: hipLaunchKernelGGL((k_eval_smfmac_occ2<1, 7, NW>), dim3(blocks), dim3(64 * NW), 0, st, d, pre, n); break;
+        default: hipLaunchKernelGGL((k_eval_smfmac_occ2<1, 8, NW>), dim3(blocks), dim3(64 * NW), 0, st, d, pre, n); break;
+    }
+}
+template <int NW>
+static void launch_mfma_occ2(int ptb, const qsft_eval_mfma_desc* d, const int32_t* pre, int n, int blocks,
+                             hipStream_t st) {
+    switch (ptb) {
+        case 5: hipLaunchKernelGGL((k_eval_mfma_occ2<0, 1, 5, NW>), dim3(blocks), dim3(64 * NW), 0, st, d, pre, n); break;
+        case 6: hipLaunchKernelGGL((k_eval_mfma_occ2<0, 1, 6, NW>), dim3(blocks), dim3(64 * NW), 0, st, d, pre, n); break;
+        case 7: hipLaunchKernelGGL((k_eval_mfma_occ2<0, 1, 7, NW>), dim3(blocks), dim3(64 * NW), 0, st, d, pre, n); break;
+        default: hipLaunchKernelGGL((k_eval_mfma_occ2<0, 1, 8, NW>), dim3(blocks), dim3(64 * NW), 0, st, d, pre, n); break;
+    }
+}
+
+// rows: base queries per workgroup (mfma_rows_per_block of the launch's descriptors)
+static hipError_t launch_smfmac(int nkb, int ptb, int rows, const qsft_eval_mfma_desc* d, const int32_t* pre, int n,
+                                int blocks, hipStream_t st) {
+    switch (nkb) {
+        case 1:
+            if (rows == 2 * kMfThreads) launch_smfmac_occ2<8>(ptb, d, pre, n, blocks, st);
+            else launch_smfmac_occ2<4>(ptb, d, pre, n, blocks, st);
+            break;
+        case 2: hipLaunchKernelGGL((k_eval_smfmac<2>), dim3(blocks), dim3(kMfThreads), 0, st, d, pre, n); break;
+        case 3: hipLaunchKernelGGL((k_eval_smfmac<3>), dim3(blocks), dim3(kMfThreads), 0, st, d, pre, n); break;
+        default: hipLaunchKernelGGL((k_eval_smfmac<4>), dim3(blocks), dim3(kMfThreads), 0, st, d, pre, n); break;
+    }
+    return hipGetLastError();
+}
+
 template <int MODE>
-static hipError_t launch_mfma(int nkb, int ptb, const qsft_eval_mfma_desc* d, const int32_t* pre, int n, int blocks,
-                              hipStream_t st) {
+static hipError_t launch_mfma(int nkb, int ptb, int rows, const qsft_eval_mfma_desc* d, const int32_t* pre, int n,
+                              int blocks, hipStream_t st) {
     switch (nkb) {
         case 1:
             if constexpr (MODE == 0) {
-                switch (ptb) {
-                    case 5: hipLaunchKernelGGL((k_eval_mfma_occ2<0, 1, 5>), dim3(blocks), dim3(kMfThreads), 0, st, d, pre, n); break;
-                    case 6: hipLaunchKernelGGL((k_eval_mfma_occ2<0, 1, 6>), dim3(blocks), dim3(kMfThreads), 0, st, d, pre, n); break;
-                    case 7: hipLaunchKernelGGL((k_eval_mfma_occ2<0, 1, 7>), dim3(blocks), dim3(kMfThreads), 0, st, d, pre, n); break;
-                    default: hipLaunchKernelGGL((k_eval_mfma_occ2<0, 1, 8>), dim3(blocks), dim3(kMfThreads), 0, st, d, pre, n); break;
-                }
+                if (rows == 2 * kMfThreads) launch_mfma_occ2<8>(ptb, d, pre, n, blocks, st);
+                else launch_mfma_occ2<4>(ptb, d, pre, n, blocks, st);
             } else {
                 hipLaunchKernelGGL((k_eval_mfma<MODE, 1>), dim3(blocks), dim3(kMfThreads), 0, st, d, pre, n);
             }
@@ -559,8 +848,9 @@ static hipError_t launch_mfma(int nkb, int ptb, const qsft_eval_mfma_desc* d, co
 }
 
 // word: bits 4..7 digit-block count (1..4), bits 8..9 mode (0: q in {2,4}; 1: other powers of two; 2: other q),
-// bits 12..15 p-tiles per workgroup (the descriptors' ptb; 5..8 for mode 0 with one digit block, else 8).
-// All instances of one launch share the word.
+// bit 10: mode 0 in the dense form (default: sparse), bit 11: eight-wave workgroups (mfma_wide of the
+// descriptors), bits 12..15 p-tiles per workgroup (the descriptors' ptb; 5..8 for mode 0 with one digit
+// block, else 8). All instances of one launch share the word.
 extern "C" int qsft_eval_mfma(const qsft_eval_mfma_desc* d, const int32_t* pre_prep, const int32_t* pre_main,
                               int n_inst, int prep_blocks, int main_blocks, int word, void* stream) {
     if (!d || !pre_prep || !pre_main || n_inst < 1 || prep_blocks < 0 || main_blocks < 0) {
@@ -568,21 +858,27 @@ extern "C" int qsft_eval_mfma(const qsft_eval_mfma_desc* d, const int32_t* pre_p
         return QSFT_ERR_ARG;
     }
     const int nkb = (word >> 4) & 0xF, mode = (word >> 8) & 0x3, ptb = (word >> 12) & 0xF;
+    const bool dense0 = (word >> 10) & 1, wide = (word >> 11) & 1;
+    const int rows = wide ? 2 * kMfThreads : kMfThreads;
     const bool split_ok = (mode == 0 && nkb == 1) ? (ptb >= 5 && ptb <= kPtPerBlock) : (ptb == kPtPerBlock);
-    if (nkb < 1 || nkb > 4 || mode > 2 || !split_ok) {
+    if (nkb < 1 || nkb > 4 || mode > 2 || !split_ok || (wide && (mode != 0 || nkb != 1))) {
         qsft_set_error("qsft_eval_mfma: bad word");
         return QSFT_ERR_ARG;
     }
     hipStream_t st = (hipStream_t)stream;
     if (prep_blocks > 0) {
-        hipLaunchKernelGGL(k_eval_mfma_prep, dim3(prep_blocks), dim3(256), 0, st, d, pre_prep, n_inst);
+        if (mode == 0 && !dense0)
+            hipLaunchKernelGGL(k_eval_mfma_prep<true>, dim3(prep_blocks), dim3(256), 0, st, d, pre_prep, n_inst);
+        else
+            hipLaunchKernelGGL(k_eval_mfma_prep<false>, dim3(prep_blocks), dim3(256), 0, st, d, pre_prep, n_inst);
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) return qsft_hip_status(e, "qsft_eval_mfma(prep)");
     }
     if (main_blocks == 0) return QSFT_OK;
     hipError_t e;
-    if (mode == 0) e = launch_mfma<0>(nkb, ptb, d, pre_main, n_inst, main_blocks, st);
-    else if (mode == 1) e = launch_mfma<1>(nkb, ptb, d, pre_main, n_inst, main_blocks, st);
-    else e = launch_mfma<2>(nkb, ptb, d, pre_main, n_inst, main_blocks, st);
+    if (mode == 0 && !dense0) e = launch_smfmac(nkb, ptb, rows, d, pre_main, n_inst, main_blocks, st);
+    else if (mode == 0) e = launch_mfma<0>(nkb, ptb, rows, d, pre_main, n_inst, main_blocks, st);
+    else if (mode == 1) e = launch_mfma<1>(nkb, ptb, rows, d, pre_main, n_inst, main_blocks, st);
+    else e = launch_mfma<2>(nkb, ptb, rows, d, pre_main, n_inst, main_blocks, st);
     return qsft_hip_status(e, "qsft_eval_mfma");
 }

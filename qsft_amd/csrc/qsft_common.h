@@ -43,6 +43,13 @@ __host__ __device__ __forceinline__ int ipow(int q, int e) {
     return r;
 }
 
+// The matrix-core evaluation runs eight-wave workgroups (512 base queries)
+// for q in {2, 4} with one digit block (n_pad = 64) and q^b >= 512, four-wave
+// ones (256) otherwise; bit 11 of its launch word says which.
+__host__ __device__ __forceinline__ bool mfma_wide(int q, int b, int n_pad) {
+    return (q == 2 || q == 4) && n_pad == 64 && ipow(q, b) >= 512;
+}
+
 // x mod q for 0 <= x < 2^32 / q, by multiply-high with m = floor(2^32 / q) + 1:
 // floor(x m / 2^32) = floor(x / q + x (m - 2^32/q) / 2^32) and the error term is
 // < x / 2^32 < 1 / q, which cannot cross the next integer.
@@ -238,6 +245,20 @@ __device__ __forceinline__ uint64_t uniform_base(const void* sbase) {
     const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(bv >> 32));
     const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)bv);
     return ((uint64_t)hi << 32) | (uint64_t)lo;
+}
+
+// dma16 with the global base already wave-uniform (uniform_base)
+__device__ __forceinline__ void dma16_u(uint64_t bs, uint32_t voff, uint32_t lds_base) {
+    uint32_t saved;
+    asm volatile(
+        "s_mov_b32 %0, m0\n\t"
+        "s_mov_b32 m0, %3\n\t"
+        "s_nop 0\n\t"
+        "global_load_lds_dwordx4 %1, %2\n\t"
+        "s_mov_b32 m0, %0"
+        : "=&s"(saved)
+        : "v"(voff), "s"(bs), "s"(lds_base)
+        : "memory");
 }
 
 __device__ __forceinline__ void dma16(const void* sbase, uint32_t voff, uint32_t lds_base) {

@@ -69,11 +69,12 @@ def evaluate_cosets(q, n, b, Mt, D, Kd, a, precision):
     return x
 
 
-def evaluate_cosets_mfma(q, n, b, Mt, D, Kd, a64):
-    """The same samples through the matrix-core kernel -> complex64 [C][P][q^b]."""
+def evaluate_cosets_mfma(q, n, b, Mt, D, Kd, a64, dense=False):
+    """The same samples through the matrix-core kernel -> complex64 [C][P][q^b]
+    (dense=True: the dense-MFMA form for q in {2, 4})."""
     C, P = Mt.shape[0], D.shape[1]
     x = torch.empty((C, P, q ** b), dtype=torch.complex64, device=Mt.device)
-    ev = K.EvalMfma([dict(q=q, n=n, b=b, C=C, P=P, S=Kd.shape[0], Mt=Mt, D=D, K=Kd, a64=a64, x=x)])
+    ev = K.EvalMfma([dict(q=q, n=n, b=b, C=C, P=P, S=Kd.shape[0], Mt=Mt, D=D, K=Kd, a64=a64, x=x)], dense=dense)
     ev.launch()
     torch.cuda.current_stream().synchronize()      # the B-fragment workspace is released with `ev`
     return x

@@ -116,6 +116,12 @@ def mfma_mode(q):
     return 1 if is_pow2(q) else 2
 
 
+def mfma_wide(q, b, n_pad):
+    """Eight-wave workgroups (512 base queries) of the matrix-core evaluator:
+    q in {2, 4}, one digit block and q^b >= 512 (bit 11 of the launch word)."""
+    return q in (2, 4) and n_pad == 64 and q ** b >= 512
+
+
 def mfma_ptb(mode, nkb, npt):
     """p-tiles per workgroup of the matrix-core evaluator: the tiles split as
     evenly as possible over ceil(npt / 8) workgroups (5..8) for q in {2, 4}
@@ -140,18 +146,21 @@ class EvalMfma:
     items: dicts q, n, b, C, P, S, Mt, D, K, a64 (complex128 [S]), x (complex64 [C][P][q^b]),
     optional amax (max |a_s|, else read from a64) and shared_d.
     The B-fragment workspace is allocated here, one per instance.
+    q in {2, 4} runs the sparse-MFMA form; dense=True selects the dense form
+    of the same products instead (for comparisons).
     """
 
-    def __init__(self, items):
+    def __init__(self, items, dense=False):
         groups = defaultdict(list)
         self.empty = [it["x"] for it in items if it["S"] == 0]     # no support: x = 0
         for it in items:
             if it["S"] > 0:
                 nkb, mode = it["Mt"].shape[-1] // 64, mfma_mode(it["q"])
-                groups[(nkb, mode, mfma_ptb(mode, nkb, (it["P"] + 7) // 8))].append(it)
+                wide = mfma_wide(it["q"], it["b"], it["Mt"].shape[-1])
+                groups[(nkb, mode, mfma_ptb(mode, nkb, (it["P"] + 7) // 8), wide)].append(it)
         self.launches = []
         self.ws = []
-        for (nkb, mode, ptb), its in groups.items():
+        for (nkb, mode, ptb, wide), its in groups.items():
             descs, prep, main = [], [], []
             for it in its:
                 if it["x"].dtype != torch.complex64 or it["a64"].dtype != torch.complex128:
@@ -176,7 +185,8 @@ class EvalMfma:
                 prep.append(pb.value)
                 main.append(mb.value)
             buf, dptr, (pp, pm), (tp, tm) = pack(descs, [prep, main], its[0]["x"].device)
-            self.launches.append((buf, dptr, pp, pm, len(descs), tp, tm, (nkb << 4) | (mode << 8) | (ptb << 12)))
+            word = (nkb << 4) | (mode << 8) | (ptb << 12) | (int(bool(dense) and mode == 0) << 10) | (int(wide) << 11)
+            self.launches.append((buf, dptr, pp, pm, len(descs), tp, tm, word))
 
     def launch(self):
         for x in self.empty:

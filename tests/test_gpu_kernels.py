@@ -118,6 +118,36 @@ def test_eval_mfma_strength_scale(q, scale):
         assert rel_err(x[c], ref) < 2e-6, c
 
 
+# support counts around the 32-support blocks of the sparse form (an odd
+# number of 16-support blocks pads a zero half-block), one to four digit
+# blocks, p-tile splits 5..8 and the generic-ptb kernels
+SPARSE_CASES = [(4, 10, 3, 1, 2, 1), (4, 10, 3, 16, 2, 2), (4, 20, 4, 17, 3, 2), (4, 30, 4, 32, 2, 3),
+                (4, 50, 5, 33, 3, 2), (2, 40, 6, 95, 2, 2), (4, 64, 4, 200, 2, 2), (4, 100, 4, 70, 2, 2),
+                (2, 200, 5, 50, 2, 1), (4, 12, 4, 64, 3, 11)]
+
+
+@pytest.mark.parametrize("q,n,b,S,C,R", SPARSE_CASES)
+def test_eval_mfma_sparse_matches_dense_and_oracle(q, n, b, S, C, R):
+    """q in {2, 4}: the sparse-MFMA form (compressed +-1 W, 2-bit row
+    indices) against the dense form of the same fp16 products and against
+    the oracle's FP64 evaluation, every row."""
+    print(f"eval_mfma sparse q={q} n={n} b={b} S={S} C={C} R={R}", flush=True)
+    Ms, D, locq, a = _setup(q, n, b, S, C, R, seed=q * 31 + n + S)
+    dev = dv.current_device()
+    n_pad = dv.n_pad_of(n)
+    args = (q, n, b, dv.Mt_tensor(Ms, b, n_pad, dev), dv.D_tensor([D] * C, n_pad, dev),
+            dv.digits_tensor(locq.T, n_pad, dev), dv.strengths_tensor(a, "fp64", dev))
+    xs = dv.evaluate_cosets_mfma(*args).cpu().numpy()
+    xd = dv.evaluate_cosets_mfma(*args, dense=True).cpu().numpy()
+    assert rel_err(xs, xd) < 1e-6
+    P = D.shape[0]
+    for c in range(C):
+        dig = O.coset_digits(Ms[c], D, q, b)
+        ref = O.eval_synthetic(dig.reshape(-1, n), locq, a, q).reshape(P, q ** b)
+        assert rel_err(xs[c], ref) < 2e-6, c
+        assert rel_err(xd[c], ref) < 2e-6, c
+
+
 @pytest.mark.parametrize("q", [4, 3])
 def test_eval_mfma_distinct_delays_per_group(q):
     """Groups with their own delay rows (B computed per group)."""
