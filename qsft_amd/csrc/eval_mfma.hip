@@ -250,8 +250,9 @@ template <int MODE, int NKB, int NPT, bool SP, int NW>
 __device__ __forceinline__ void eval_mfma_body(const qsft_eval_mfma_desc* __restrict__ descs,
                                                const int32_t* __restrict__ pre, int n_inst) {
     static_assert(!SP || MODE == 0, "the sparse form needs w^e in {1, i, -1, -i}");
-    constexpr int kThreads = 64 * NW;                // NW waves of 64 base queries each
-    constexpr int kRowsPerBlock = kThreads;          // base queries per workgroup
+    constexpr int TT = 4;                            // subtiles of 16 base queries per wave
+    constexpr int kThreads = 64 * NW;                // NW waves of TT subtiles
+    constexpr int kRowsPerBlock = 16 * TT * NW;      // base queries per workgroup
     constexpr int kStagePerThread = kStage / kThreads;   // B loads per lane per slot
     constexpr int WL = (MODE == 0) ? 1 : kLimbs;
     __shared__ __align__(16) v4u bs0[kStage], bs1[kStage], bs2[kStage], bs3[kStage];   // 4 x 16 KiB of B fragments
@@ -262,7 +263,7 @@ __device__ __forceinline__ void eval_mfma_body(const qsft_eval_mfma_desc* __rest
     const int inst = find_inst(pre, n_inst, lblock);
     const qsft_eval_mfma_desc& d = descs[inst];
     if (d.ptb != NPT) return;                                      // launched for another split (host error)
-    if (mfma_wide(d.q, d.b, d.n_pad) != (NW == 8)) return;         // ... or another workgroup size
+    if (mfma_wide(d.q, d.b, d.n_pad) != (kRowsPerBlock == 512)) return;   // ... or another workgroup size
     const int q = d.q, b = d.b, n_pad = d.n_pad, npt = d.npt, P = d.P, S = d.S;
     const int nsb = SP ? 2 * ((d.nsb + 1) / 2) : d.nsb;          // 16-support blocks (even when sparse)
     const int Q = ipow(q, b);
@@ -295,13 +296,13 @@ __device__ __forceinline__ void eval_mfma_body(const qsft_eval_mfma_desc* __rest
     const uint32_t wmask = (q == 2) ? 0x01000100u : 0x03000300u;
     const uint32_t wre = (q == 2) ? 0x0000BC3Cu : 0x00BC003Cu;
     const uint32_t wim = (q == 2) ? 0u : 0xBC003C00u;
-    const int lbase = lb * kRowsPerBlock + wave * 64;
+    const int lbase = lb * kRowsPerBlock + wave * 16 * TT;
 
     // base-query digit fragments (B operand of the int8 MFMA): column l = lbase + 16 t + r16
-    v4i bq[4][NKB];
+    v4i bq[TT][NKB];
     const int8_t* __restrict__ Mt = d.Mt + (size_t)c * b * n_pad;
 #pragma unroll
-    for (int t = 0; t < 4; ++t) {
+    for (int t = 0; t < TT; ++t) {
         const int l = lbase + 16 * t + r16;
 #pragma unroll
         for (int kb = 0; kb < NKB; ++kb) {
@@ -331,9 +332,9 @@ __device__ __forceinline__ void eval_mfma_body(const qsft_eval_mfma_desc* __rest
         }
     }
 
-    v4f acc[4][NPT];
+    v4f acc[TT][NPT];
 #pragma unroll
-    for (int t = 0; t < 4; ++t)
+    for (int t = 0; t < TT; ++t)
 #pragma unroll
         for (int pt = 0; pt < NPT; ++pt) acc[t][pt] = v4f{0.f, 0.f, 0.f, 0.f};
 
@@ -528,30 +529,29 @@ __device__ __forceinline__ void eval_mfma_body(const qsft_eval_mfma_desc* __rest
         }
     };
     // all W fragments of a block at once (the first block)
-    auto sp_wall = [&](const v4i* k0, const v4i* k1, v4u (&wa)[4], uint32_t (&wi)[4]) {
+    auto sp_wall = [&](const v4i* k0, const v4i* k1, v4u (&wa)[TT], uint32_t (&wi)[TT]) {
         v4i kf0[NKB], kf1[NKB];
 #pragma unroll
         for (int kb = 0; kb < NKB; ++kb) {
             kf0[kb] = k0[kb * 64 + lane];
             kf1[kb] = k1[kb * 64 + lane];
         }
-        v4i ea[4], eb[4];
+        v4i ea[TT], eb[TT];
 #pragma unroll
-        for (int t = 0; t < 4; ++t) {
+        for (int t = 0; t < TT; ++t) {
             ea[t] = e0_tile(kf0, t);
             eb[t] = e0_tile(kf1, t);
         }
 #pragma unroll
-        for (int t = 0; t < 4; ++t) sp_w(ea[t], eb[t], wa[t], wi[t]);
+        for (int t = 0; t < TT; ++t) sp_w(ea[t], eb[t], wa[t], wi[t]);
     };
     // The GEMM of one block with W fragments (wa, wi). With NEXT, the next
-    // block's W fragments (digits in kn0, kn1) are built in the first six of
+    // block's W fragments (digits in kn0, kn1) are built in the first TT + 2 of
     // the 2 NPT fragment steps: digit reads after step 0, subtile t's two int8
-    // products after step t + 1 and its byte work after step t + 2, so that
-    // the VALU work runs while this wave's MFMAs execute; scheduling barriers
-    // keep that order.
-    auto sp_gemm = [&](const v4u* b0, const v4u* b1, const v4u (&wa)[4], const uint32_t (&wi)[4], const v4i* kn0,
-                       const v4i* kn1, v4u (&wn)[4], uint32_t (&win)[4], auto next) {
+    // products after step t + 1 and its byte work after step t + 2, spread
+    // between this wave's MFMAs; scheduling barriers keep that order.
+    auto sp_gemm = [&](const v4u* b0, const v4u* b1, const v4u (&wa)[TT], const uint32_t (&wi)[TT], const v4i* kn0,
+                       const v4i* kn1, v4u (&wn)[TT], uint32_t (&win)[TT], auto next) {
         constexpr bool NEXT = decltype(next)::value;
         auto frag = [&](int f, v4u& lo, v4u& hi) {
             const v4u* base = (f < 8) ? b0 : b1;
@@ -570,7 +570,7 @@ __device__ __forceinline__ void eval_mfma_body(const qsft_eval_mfma_desc* __rest
             if (st + 1 < kLimbs * NPT) frag(2 * ((st + 1) % NPT) + (st + 1) / NPT, nlo, nhi);
             __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-            for (int t = 0; t < 4; ++t)
+            for (int t = 0; t < TT; ++t)
                 acc[t][pt] = __builtin_amdgcn_smfmac_f32_16x16x64_f16(__builtin_bit_cast(v8h, wa[t]), bf, acc[t][pt],
                                                                      (int)wi[t], 0, 0);
             if constexpr (NEXT) {
@@ -581,12 +581,12 @@ __device__ __forceinline__ void eval_mfma_body(const qsft_eval_mfma_desc* __rest
                         kf1[kb] = kn1[kb * 64 + lane];
                     }
                 }
-                if (st >= 2 && st <= 5) {
+                if (st >= 2 && st <= TT + 1) {
                     const int t = st - 2;
                     sp_w(ea, eb, wn[t], win[t]);
                     __asm__ volatile("" : "+v"(wn[t]), "+v"(win[t]));   // materialise here
                 }
-                if (st >= 1 && st <= 4) {
+                if (st >= 1 && st <= TT) {
                     ea = e0_tile(kf0, st - 1);
                     eb = e0_tile(kf1, st - 1);
                 }
@@ -603,8 +603,8 @@ __device__ __forceinline__ void eval_mfma_body(const qsft_eval_mfma_desc* __rest
         if (nsp > 1) issue_kd(1, ks2, ks3);
         wait_vm<0>();
         ring_barrier();
-        v4u wa0[4], wa1[4];
-        uint32_t wi0[4], wi1[4];
+        v4u wa0[TT], wa1[TT];
+        uint32_t wi0[TT], wi1[TT];
         sp_wall(ks0, ks1, wa0, wi0);
         wait_lgkm0();
         ring_barrier();                              // every wave is done with block 0's digit slots
@@ -689,7 +689,7 @@ __device__ __forceinline__ void eval_mfma_body(const qsft_eval_mfma_desc* __rest
     const bool even_q = (Q & 1) == 0;
     const float unscale = ldexpf(1.0f, -d.bexp);                  // exact: 2^-bexp is a normal float
 #pragma unroll
-    for (int t = 0; t < 4; ++t) {
+    for (int t = 0; t < TT; ++t) {
         const int l0 = lbase + 16 * t + 4 * g + (odd ? 2 : 0);
 #pragma unroll
         for (int pt = 0; pt < NPT; ++pt) {
