@@ -9,11 +9,12 @@ detection, 15-pass cap.
 `value` is the serving loop's throughput on that workload: `--steps x
 --batch` distinct requests (each with its own support, strengths, M_c,
 delays and noise seed; the requests carry their digits as int8 arrays) are
-submitted at once to a `qsft_amd.serving.Server` (admission by HBM
+submitted back to back to a `qsft_amd.serving.Server` (admission by HBM
 footprint, continuous batching of up to --batch requests over
 --inflight streams, per-request delivery of the recovered rows into pinned
-host memory, in the reference's dict order), timed from the first submit to
-the last result. A "step" is one batch of --batch requests. Latency p50/p99
+host memory, in the reference's dict order), the server polling after every
+--batch submissions so that batches start while the burst is still being
+submitted; timed from the first submit to the last result. A "step" is one batch of --batch requests. Latency p50/p99
 come from a second, open-loop run with Poisson arrivals at 80 % of that rate
 (per request, arrival -> result).
 
@@ -59,7 +60,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=40, help="batches of --batch requests in the timed serving run")
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--batch", type=int, default=64)
-    ap.add_argument("--inflight", type=int, default=3, help="batches on the GPU at once (one stream each)")
+    ap.add_argument("--inflight", type=int, default=4, help="batches on the GPU at once (one stream each)")
     ap.add_argument("--plain", type=int, default=3, help="plain FP64 transforms to time")
     ap.add_argument("--seed", type=int, default=1234)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -174,14 +175,20 @@ def mix_specs(count, rng, large=0.05):
 
 
 def serve_closed(specs, batch, inflight, budget):
-    """Every request submitted at once; transforms/s from first submit to last result."""
+    """A burst of requests submitted back to back, the server polling after
+    every `batch` submissions (host submission overlaps the GPU work of the
+    batches already formed); transforms/s from first submit to last result."""
     import torch
 
     from qsft_amd import serving as SV
     srv = SV.Server(hbm_budget=budget, max_batch=batch, max_inflight=inflight)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    reqs = [srv.submit(sp) for sp in specs]
+    reqs = []
+    for i, sp in enumerate(specs):
+        reqs.append(srv.submit(sp))
+        if (i + 1) % batch == 0:
+            srv.poll()
     srv.drain()
     el = time.perf_counter() - t0
     return reqs, el, srv
