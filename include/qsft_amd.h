@@ -439,7 +439,8 @@ typedef struct qsft_request {
     int32_t _pad0;
     double cutoff;              /* zeroton threshold per row (reference: 1e-9 + 1.5 noise_sd^2 / q^b) */
     double sigma;               /* transform-domain noise sd per bin (noise_sd / sqrt(q^b)); 0: none */
-    double amax;                /* max |a_s| (> 0)                                              */
+    double amax;                /* max |a_s| (> 0), or 0: computed from the strengths when the   *
+                                 * request is packed (qsft_batch_pack_input)                    */
     uint64_t seed;              /* Philox key of the request's noise                            */
 } qsft_request;
 

@@ -53,7 +53,7 @@ bool layout_of(const qsft_request& r, int max_iter, ReqLayout* L) {
     if (r.q < 2 || r.q > QSFT_MAX_Q || r.n < 1 || r.n > QSFT_MAX_N || r.b < 1 || r.b > QSFT_MAX_B || r.C < 1 ||
         r.C > QSFT_MAX_C || r.R < 1 || r.R > QSFT_MAX_R || r.S < 1 ||
         (r.method != QSFT_DETECT_NSO1 && r.method != QSFT_DETECT_NOISELESS) ||
-        (r.method == QSFT_DETECT_NOISELESS && r.R != 1) || !(r.sigma >= 0.0) || !(r.amax > 0.0))
+        (r.method == QSFT_DETECT_NOISELESS && r.R != 1) || !(r.sigma >= 0.0) || !(r.amax >= 0.0))
         return false;
     ReqLayout l{};
     l.n_pad = (r.n + 63) / 64 * 64;
@@ -511,11 +511,28 @@ extern "C" int qsft_batch_pack_input(void* h, int i, const int8_t* M, const int8
         qsft_set_error("qsft_batch_pack_input: bad arguments");
         return QSFT_ERR_ARG;
     }
-    const qsft_request& r = B->req[i];
+    qsft_request& r = B->req[i];
     const ReqLayout& l = B->lay[i];
     // locals only in the loops: int8 stores may alias anything, so fields read
     // through references would be reloaded on every store
     const int n = r.n, b = r.b, C = r.C, P = l.P, S = r.S, np = l.n_pad;
+    if (!(r.amax > 0.0)) {
+        // amax 0: max |a_s| from the strengths, as sqrt(max(re^2 + im^2)) with
+        // every product and sum rounded (no fused multiply-add)
+        double m2 = 0.0;
+        bool nan = false;
+        for (int s = 0; s < S; ++s) {
+            const double re = a[2 * (size_t)s], im = a[2 * (size_t)s + 1];
+            const double t = __builtin_fma(re, re, 0.0) + __builtin_fma(im, im, 0.0);   // each product rounded
+            nan |= t != t;
+            m2 = t > m2 ? t : m2;
+        }
+        if (nan || !(m2 > 0.0) || m2 == __builtin_inf()) {
+            qsft_set_error("qsft_batch_pack_input: the strengths are all zero or not finite");
+            return QSFT_ERR_ARG;
+        }
+        r.amax = __builtin_sqrt(m2);
+    }
     const int q = r.q;
     char* base = static_cast<char*>(input_host) + l.in_off;
     int8_t* __restrict__ Mt = reinterpret_cast<int8_t*>(base + l.mt);

@@ -15,6 +15,7 @@ of all recovered rows. Uneven instances (different q, n, b, S, C, R) share
 the same launches.
 """
 import collections
+import ctypes
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -24,6 +25,15 @@ from . import _lib as L
 from . import device as dv
 from . import kernels as K
 
+
+
+def _addr(x):
+    """Address of a C-contiguous array's data (ctypes.from_buffer is the
+    cheaper route for the writable arrays wire() builds)."""
+    try:
+        return ctypes.addressof(ctypes.c_char.from_buffer(x))
+    except (TypeError, ValueError):
+        return int(x.ctypes.data)
 
 @dataclass
 class TransformSpec:
@@ -70,6 +80,14 @@ class TransformSpec:
                  np.ascontiguousarray(self.strengths, dtype=np.complex128))
             self._wire = w
         return w
+
+    def wire_ptrs(self):
+        """Addresses of the four wire() arrays (kept alive by the spec)."""
+        p = getattr(self, "_wire_ptrs", None)
+        if p is None:
+            p = tuple(_addr(x) for x in self.wire())
+            self._wire_ptrs = p
+        return p
 
     def footprint_bytes(self, precision="fp32"):
         """HBM bytes this transform holds while resident (samples/transform, FP64

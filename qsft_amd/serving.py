@@ -55,10 +55,9 @@ def request_struct(sp, max_iter=15):
     else:
         raise ValueError(f"detection method {sp.method!r}")
     Q = sp.q ** sp.b
-    a = sp.strengths
-    amax = float(np.sqrt(np.max(a.real * a.real + a.imag * a.imag))) if a.size else 0.0
-    return L.Request(sp.q, sp.n, sp.b, sp.C, sp.R, a.size, method, 0, float(sp.threshold()),
-                     float(sp.noise_sd) / Q ** 0.5 if sp.noise_sd > 0 else 0.0, amax,
+    # amax 0: the runtime takes max |a_s| from the strengths while it packs them
+    return L.Request(sp.q, sp.n, sp.b, sp.C, sp.R, sp.strengths.size, method, 0, float(sp.threshold()),
+                     float(sp.noise_sd) / Q ** 0.5 if sp.noise_sd > 0 else 0.0, 0.0,
                      int(sp.seed) & 0xFFFFFFFFFFFFFFFF)
 
 
@@ -108,13 +107,11 @@ class ServedBatch:
         self.work_dev = torch.empty(max(int(sz.work_bytes), 256), dtype=torch.uint8, device=device)
         self.plan_dev = torch.empty(max(int(sz.plan_bytes), 256), dtype=torch.uint8, device=device)
         base = self.in_host.data_ptr()
-        ptrs = [(ctypes.c_void_p * n)() for _ in range(4)]
-        for i, sp in enumerate(self.specs):
-            w = sp.wire()
-            for j in range(4):
-                ptrs[j][i] = w[j].ctypes.data
-        L.check(lib.qsft_batch_pack_inputs(self.h, *ptrs, base, PACK_THREADS if n >= 8 else 1),
-                "qsft_batch_pack_inputs")
+        # [4][n] addresses of every request's M, D, locq and strengths
+        ptrs = np.ascontiguousarray(np.array([sp.wire_ptrs() for sp in self.specs], dtype=np.uint64).T)
+        p0 = ptrs.ctypes.data
+        L.check(lib.qsft_batch_pack_inputs(self.h, p0, p0 + 8 * n, p0 + 16 * n, p0 + 24 * n, base,
+                                           PACK_THREADS if n >= 8 else 1), "qsft_batch_pack_inputs")
         L.check(lib.qsft_batch_bind(self.h, self.in_dev.data_ptr(), self.work_dev.data_ptr(),
                                     self.plan_host.data_ptr(), self.plan_dev.data_ptr(), self.out.data_ptr()),
                 "qsft_batch_bind")
