@@ -327,6 +327,8 @@ typedef struct qsft_peel_desc {
                                  * and the identity delays (s = +1); signatures then take *
                                  * R dot products per key instead of P                   */
 #define QSFT_PEEL_UNIT_D_NEG 4  /* with QSFT_PEEL_UNIT_D: s = -1                          */
+#define QSFT_PEEL_ONE_BIN_WAVES 16 /* first pass with one bin per wave (default: four   *
+                                  * bins per wave where possible; same results)          */
 #define QSFT_PEEL_SPARSE_U 8    /* U holds only the columns the assembler wrote (above;  *
                                  * ignored unless xsrc, written and noise_key are set)   */
 

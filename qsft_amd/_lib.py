@@ -24,6 +24,7 @@ QSFT_DEVERR_TABLE_FULL, QSFT_DEVERR_TAG_COLLISION, QSFT_DEVERR_OUT_FULL = 1, 2, 
 QSFT_MAX_Q, QSFT_MAX_N, QSFT_MAX_B, QSFT_MAX_C, QSFT_MAX_R = 64, 256, 16, 16, 16
 QSFT_DFT_MAX_PASSES = 4
 QSFT_PEEL_SHARED_D, QSFT_PEEL_UNIT_D, QSFT_PEEL_UNIT_D_NEG, QSFT_PEEL_SPARSE_U = 1, 2, 4, 8
+QSFT_PEEL_ONE_BIN_WAVES = 16
 QSFT_CTR_ITER_STRIDE = 8
 QSFT_CTR_LOG, QSFT_CTR_OUT, QSFT_CTR_ERROR, QSFT_CTR_ITERS, QSFT_CTR_PEELED, QSFT_CTR_LIVE = 128, 129, 130, 131, 132, 133
 QSFT_FIN_FLUSH = 1

@@ -587,6 +587,48 @@ __device__ __forceinline__ uint32_t noiseless_digit(double2 v0, double2 vi, int 
     return noiseless_slow(v0, vi, q);
 }
 
+// The detection's FP64 accumulations, with every rounding written out (no
+// contraction left to the compiler), so that the one-bin-per-wave and the
+// four-bins-per-wave detections give the same bits:
+// rr += Re(conj(w) u), ri += Im(conj(w) u); s += |u - rho w|^2.
+__device__ __forceinline__ void conj_dot_acc(const double2 w, const double2 u, double& rr, double& ri) {
+    rr = __dadd_rn(rr, __fma_rn(w.x, u.x, __dmul_rn(w.y, u.y)));
+    ri = __dadd_rn(ri, __fma_rn(w.x, u.y, -__dmul_rn(w.y, u.x)));
+}
+__device__ __forceinline__ void residual_acc(const double2 u, const double2 w, double rho_x, double rho_y,
+                                             double& s) {
+    const double dx = __dsub_rn(u.x, __fma_rn(rho_x, w.x, -__dmul_rn(rho_y, w.y)));
+    const double dy = __dsub_rn(u.y, __fma_rn(rho_x, w.y, __dmul_rn(rho_y, w.x)));
+    s = __dadd_rn(s, __fma_rn(dx, dx, __dmul_rn(dy, dy)));
+}
+
+// Digit of coordinate i (1..n) of the k of a bin from its column cw (rows in
+// LDS): nso1 (the mean over the R repeats of u_{r,0} conj(u_{r,i})) or the
+// noiseless rule.
+__device__ __forceinline__ uint32_t coord_digit(const qsft_peel_desc& d, const double2* cw, int i, int q, int P) {
+    if (d.method & QSFT_DETECT_NSO1) {
+        const int p1 = d.p1, R = P / p1;
+        double zr = 0.0, zi = 0.0;
+        for (int r = 0; r < R; ++r) {
+            const double2 u0 = cw[r * p1];
+            const double2 ui = cw[r * p1 + i];
+            // u0 * conj(ui), as numpy forms the complex product
+            const double pr = __dsub_rn(__dmul_rn(u0.x, ui.x), __dmul_rn(u0.y, -ui.y));
+            const double pi_ = __dadd_rn(__dmul_rn(u0.x, -ui.y), __dmul_rn(u0.y, ui.x));
+            zr = __dadd_rn(zr, pr);
+            zi = __dadd_rn(zi, pi_);
+        }
+        return nso1_digit(zr, zi, q, R);
+    }
+    return noiseless_digit(cw[0], cw[i], q);
+}
+
+// Residual energy |col - rho sig|^2 = |col|^2 - P |rho|^2 (|sig_p| = 1 and
+// sig^H col = P rho), roundings written out.
+__device__ __forceinline__ double residual_energy(double energy, int P, double rho_x, double rho_y) {
+    return __fma_rn(-(double)P, __fma_rn(rho_x, rho_x, __dmul_rn(rho_y, rho_y)), energy);
+}
+
 // Per-wave LDS of the detection: the k digits, two column slots with the
 // first ball queued on each, and the singletons buffered for registration.
 // ---- coded source: Reed-Solomon syndrome decoding in one wave
@@ -763,6 +805,83 @@ __device__ __forceinline__ uint64_t scan_marks(const qsft_peel_desc& d, int it, 
 // One wave classifies the due bins `todo` among the 64 from `base` and
 // registers its singletons (kernel A's body; the fused tail kernel runs it
 // with PERSIST). tab, offbuf and mtbuf are the block's staged tables.
+// End of a wave's detection: the pass counters (bins scanned, singletons,
+// multitons: one atomic each per wave), then the hash registration of the
+// wave's buffered singletons, one lane each, so the atomics of up to 64
+// registrations are in flight together.
+__device__ __forceinline__ void finish_wave(const qsft_peel_desc& d, int it, uint32_t scanned, int nz, int ns,
+                                            const int32_t* sb_t, const uint64_t* sb_h, const double2* sb_rho,
+                                            int lane) {
+    uint32_t* __restrict__ ctr = d.ctr;
+    const int CQ = d.C * d.Q;
+    if (lane == 0) {
+        uint32_t* pc = ctr + it * QSFT_CTR_ITER_STRIDE;
+        atomicAdd(pc + 4, scanned);
+        if (ns) atomicAdd(pc + 0, (uint32_t)ns);
+        if (nz > ns) atomicAdd(pc + 1, (uint32_t)(nz - ns));
+    }
+    if (ns == 0) return;
+
+    // --- hash registration, one lane per singleton of this wave
+    int slot = -1;
+    bool claimed = false;                                      // this lane's k took a new slot
+    bool newball = false;
+    int t = 0;
+    uint64_t tm = 0;
+    double2 rho = make_double2(0.0, 0.0);
+    if (lane < ns) {
+        t = sb_t[lane];
+        rho = sb_rho[lane];
+        const uint64_t hv = sb_h[lane];
+        const uint64_t tg = hv | 1ull;
+        const uint32_t mask = (1u << d.hcap_log2) - 1u;
+        uint32_t pos = (uint32_t)(hv >> 32) & mask;
+        for (uint32_t probe = 0; probe <= mask; ++probe) {
+            const unsigned long long old = atomicCAS((unsigned long long*)&d.tag[pos], 0ull, (unsigned long long)tg);
+            if (old == 0ull || old == tg) {
+                slot = (int)pos;
+                claimed = old == 0ull;
+                break;
+            }
+            pos = (pos + 1) & mask;
+        }
+        if (slot < 0) {
+            atomicOr(&ctr[QSFT_CTR_ERROR], (uint32_t)QSFT_DEVERR_TABLE_FULL);
+        } else {
+            d.slot_bin[t] = slot;
+            d.sstamp[t] = it + 1;                                 // kernel B peels only registered bins
+            tm = ((uint64_t)(it + 1) << 32) | (uint32_t)t;
+            atomicMin((unsigned long long*)&d.owner[slot], (unsigned long long)tm);
+            const unsigned long long prev = atomicMax((unsigned long long*)&d.last[slot], (unsigned long long)tm);
+            newball = (prev >> 32) != (unsigned long long)(it + 1);
+        }
+    }
+    const uint64_t nb_mask = __ballot(newball);
+    const uint64_t ok_mask = __ballot(slot >= 0);
+    const uint64_t cl_mask = __ballot(claimed);
+    uint32_t rbase = 0, lbase = 0;
+    if (lane == 0) {
+        if (nb_mask) atomicAdd(&ctr[it * QSFT_CTR_ITER_STRIDE + 2], (uint32_t)__popcll(nb_mask));
+        if (ok_mask) rbase = atomicAdd(&ctr[QSFT_CTR_LOG], (uint32_t)__popcll(ok_mask));
+        if (cl_mask) lbase = atomicAdd(&ctr[QSFT_CTR_LIVE], (uint32_t)__popcll(cl_mask));
+    }
+    rbase = __shfl(rbase, 0);
+    lbase = __shfl(lbase, 0);
+    if (claimed) d.live[lbase + (uint32_t)__popcll(cl_mask & ((1ull << lane) - 1ull))] = (uint32_t)slot;
+    if (slot >= 0) {
+        const uint32_t rix = rbase + (uint32_t)__popcll(ok_mask & ((1ull << lane) - 1ull));
+        if ((uint64_t)rix >= (uint64_t)d.max_iter * (uint64_t)CQ) {   // cannot happen: one record per bin per pass
+            atomicOr(&ctr[QSFT_CTR_ERROR], (uint32_t)QSFT_DEVERR_BAD_SHAPE);
+        } else {
+            d.rec_time[rix] = tm;
+            d.rec_rho[2 * (size_t)rix] = rho.x;
+            d.rec_rho[2 * (size_t)rix + 1] = rho.y;
+            d.rec_next[rix] = atomicExch(&d.head[slot], rix);
+            atomicAdd(&d.count[slot], 1u);
+        }
+    }
+}
+
 template <int NR, bool PERSIST>
 __device__ __forceinline__ void detect_bins(const qsft_peel_desc& d, int it, int base, uint64_t todo, uint32_t nheld,
                                             DetectWaveLds<NR>& W, const double2* tab, const int8_t* offbuf,
@@ -876,26 +995,7 @@ __device__ __forceinline__ void detect_bins(const qsft_peel_desc& d, int it, int
             int8_t* dig = coded ? W.syn : kb8;
             for (int i0 = 1; i0 <= nch; i0 += 64) {
                 const int i = i0 + lane;
-                if (i <= nch) {
-                    uint32_t digit;
-                    if (d.method & QSFT_DETECT_NSO1) {
-                        const int p1 = d.p1, R = P / p1;
-                        double zr = 0.0, zi = 0.0;
-                        for (int r = 0; r < R; ++r) {
-                            const double2 u0 = cw[r * p1];
-                            const double2 ui = cw[r * p1 + i];
-                            // u0 * conj(ui), as numpy forms the complex product
-                            const double pr = __dsub_rn(__dmul_rn(u0.x, ui.x), __dmul_rn(u0.y, -ui.y));
-                            const double pi_ = __dadd_rn(__dmul_rn(u0.x, -ui.y), __dmul_rn(u0.y, ui.x));
-                            zr = __dadd_rn(zr, pr);
-                            zi = __dadd_rn(zi, pi_);
-                        }
-                        digit = nso1_digit(zr, zi, q, R);
-                    } else {
-                        digit = noiseless_digit(cw[0], cw[i], q);
-                    }
-                    dig[i - 1] = (int8_t)digit;
-                }
+                if (i <= nch) dig[i - 1] = (int8_t)coord_digit(d, cw, i, q, P);
             }
             wave_lds_sync();
             if (coded) rs_decode_wave(d, W.syn, kb8, W.rs, lane);
@@ -915,10 +1015,7 @@ __device__ __forceinline__ void detect_bins(const qsft_peel_desc& d, int it, int
 #pragma unroll
             for (int r = 0; r < NR; ++r) {
                 const int p = lane + 64 * r;
-                if (p < P) {
-                    rr += w[r].x * u[r].x + w[r].y * u[r].y;       // conj(w) * u
-                    ri += w[r].x * u[r].y - w[r].y * u[r].x;
-                }
+                if (p < P) conj_dot_acc(w[r], u[r], rr, ri);       // conj(w) * u
             }
             rr = wave_sum(rr);
             ri = wave_sum(ri);
@@ -929,17 +1026,13 @@ __device__ __forceinline__ void detect_bins(const qsft_peel_desc& d, int it, int
             // could exceed 1e-3 of the threshold (|col|^2 > 1e12 thr: large
             // amplitudes against the noiseless cutoff 1e-9 P) the residual is
             // summed explicitly over the rows instead, as the reference forms it.
-            double res = energy - (double)P * (rho_x * rho_x + rho_y * rho_y);
+            double res = residual_energy(energy, P, rho_x, rho_y);
             if (energy > 1e12 * thr) {                             // wave-uniform
                 double s = 0.0;
 #pragma unroll
                 for (int r = 0; r < NR; ++r) {
                     const int p = lane + 64 * r;
-                    if (p < P) {
-                        const double dx = u[r].x - (rho_x * w[r].x - rho_y * w[r].y);
-                        const double dy = u[r].y - (rho_x * w[r].y + rho_y * w[r].x);
-                        s += dx * dx + dy * dy;
-                    }
+                    if (p < P) residual_acc(u[r], w[r], rho_x, rho_y, s);
                 }
                 res = wave_sum(s);
             }
@@ -983,74 +1076,294 @@ __device__ __forceinline__ void detect_bins(const qsft_peel_desc& d, int it, int
         wave_lds_sync();                                           // kk / this slot are rewritten next
     }
 
-    if (lane == 0) {
-        uint32_t* pc = ctr + it * QSFT_CTR_ITER_STRIDE;
-        atomicAdd(pc + 4, scanned);
-        if (ns) atomicAdd(pc + 0, (uint32_t)ns);
-        if (nz > ns) atomicAdd(pc + 1, (uint32_t)(nz - ns));
-    }
-    if (ns == 0) return;
-
-    // --- hash registration, one lane per singleton of this wave
-    int slot = -1;
-    bool claimed = false;                                      // this lane's k took a new slot
-    bool newball = false;
-    int t = 0;
-    uint64_t tm = 0;
-    double2 rho = make_double2(0.0, 0.0);
-    if (lane < ns) {
-        t = W.sb_t[lane];
-        rho = W.sb_rho[lane];
-        const uint64_t hv = W.sb_h[lane];
-        const uint64_t tg = hv | 1ull;
-        const uint32_t mask = (1u << d.hcap_log2) - 1u;
-        uint32_t pos = (uint32_t)(hv >> 32) & mask;
-        for (uint32_t probe = 0; probe <= mask; ++probe) {
-            const unsigned long long old = atomicCAS((unsigned long long*)&d.tag[pos], 0ull, (unsigned long long)tg);
-            if (old == 0ull || old == tg) {
-                slot = (int)pos;
-                claimed = old == 0ull;
-                break;
-            }
-            pos = (pos + 1) & mask;
-        }
-        if (slot < 0) {
-            atomicOr(&ctr[QSFT_CTR_ERROR], (uint32_t)QSFT_DEVERR_TABLE_FULL);
-        } else {
-            d.slot_bin[t] = slot;
-            d.sstamp[t] = it + 1;                                 // kernel B peels only registered bins
-            tm = ((uint64_t)(it + 1) << 32) | (uint32_t)t;
-            atomicMin((unsigned long long*)&d.owner[slot], (unsigned long long)tm);
-            const unsigned long long prev = atomicMax((unsigned long long*)&d.last[slot], (unsigned long long)tm);
-            newball = (prev >> 32) != (unsigned long long)(it + 1);
-        }
-    }
-    const uint64_t nb_mask = __ballot(newball);
-    const uint64_t ok_mask = __ballot(slot >= 0);
-    const uint64_t cl_mask = __ballot(claimed);
-    uint32_t rbase = 0, lbase = 0;
-    if (lane == 0) {
-        if (nb_mask) atomicAdd(&ctr[it * QSFT_CTR_ITER_STRIDE + 2], (uint32_t)__popcll(nb_mask));
-        if (ok_mask) rbase = atomicAdd(&ctr[QSFT_CTR_LOG], (uint32_t)__popcll(ok_mask));
-        if (cl_mask) lbase = atomicAdd(&ctr[QSFT_CTR_LIVE], (uint32_t)__popcll(cl_mask));
-    }
-    rbase = __shfl(rbase, 0);
-    lbase = __shfl(lbase, 0);
-    if (claimed) d.live[lbase + (uint32_t)__popcll(cl_mask & ((1ull << lane) - 1ull))] = (uint32_t)slot;
-    if (slot >= 0) {
-        const uint32_t rix = rbase + (uint32_t)__popcll(ok_mask & ((1ull << lane) - 1ull));
-        if ((uint64_t)rix >= (uint64_t)d.max_iter * (uint64_t)CQ) {   // cannot happen: one record per bin per pass
-            atomicOr(&ctr[QSFT_CTR_ERROR], (uint32_t)QSFT_DEVERR_BAD_SHAPE);
-        } else {
-            d.rec_time[rix] = tm;
-            d.rec_rho[2 * (size_t)rix] = rho.x;
-            d.rec_rho[2 * (size_t)rix + 1] = rho.y;
-            d.rec_next[rix] = atomicExch(&d.head[slot], rix);
-            atomicAdd(&d.count[slot], 1u);
-        }
-    }
+    finish_wave(d, it, scanned, nz, ns, W.sb_t, W.sb_h, W.sb_rho, lane);
 }
 
+// ---- The first pass, four bins per wave ------------------------------------
+// The first pass has no queued balls and every bin it visits was written by
+// the assembler, so a wave can classify four bins at once: 16 lanes per bin,
+// rows p = l + 16 r (r < 8, P <= 128), coordinates i = 1 + l + 16 m. One
+// instruction then serves four bins where the one-bin layout spends it on one
+// (its sums keep only 2 rows per lane busy and its dot products, hash and
+// bin match a few lanes). Every value is formed exactly as detect_bins forms
+// it: lane l of a bin holds the rows of that layout's lanes l + 16 r' (r' < 4:
+// rows l + 16 r' and l + 16 r' + 64), so its four per-lane partial sums are
+// that layout's per-lane sums, the in-row butterfly is the same DPP sequence,
+// and the four row totals combine as (R0 + R1) + (R2 + R3), as the cross-row
+// swaps of wave_sum do: energies, rho, residuals, digits, phases and hashes
+// come out bit for bit as in the one-bin layout.
+constexpr int kQuad = 4;                 // bins a wave classifies at once
+constexpr int kQRows = 8;                // rows per lane (P <= 128)
+
+struct QuadWaveLds {
+    __align__(16) double2 col[2][kQuad][128];      // current / next quad's columns
+    __align__(16) uint32_t kws[kQuad][16];         // each bin's k (n_pad = 64)
+    __align__(16) int32_t off[kQuad][4];           // each bin's <offset_r, k> mod q (unit delays)
+    uint64_t sb_h[kBinsPerWave];
+    double2 sb_rho[kBinsPerWave];
+    int32_t sb_t[kBinsPerWave];
+};
+
+// In-row (16 lanes) part of wave_sum: the same DPP butterfly.
+__device__ __forceinline__ double row16_sum(double v) {
+    v += dpp_f64<0xB1>(v);
+    v += dpp_f64<0x4E>(v);
+    v += dpp_f64<0x141>(v);
+    v += dpp_f64<0x140>(v);
+    return v;
+}
+// The four virtual rows' partial sums (one per r') to the wave_sum total.
+__device__ __forceinline__ double quad_total(const double (&s)[4]) {
+    const double r0 = row16_sum(s[0]), r1 = row16_sum(s[1]), r2 = row16_sum(s[2]), r3 = row16_sum(s[3]);
+    return (r0 + r1) + (r2 + r3);
+}
+// hash_k over the 16 words of a bin's k held by the 16 lanes of its row.
+__device__ __forceinline__ uint64_t hash_k16(uint32_t word, int l16) {
+    uint64_t v = mix64((uint64_t)word ^ ((uint64_t)(l16 + 1) * 0x9E3779B97F4A7C15ull));
+    v ^= (uint64_t)dpp_u32<0xB1>((uint32_t)v) | ((uint64_t)dpp_u32<0xB1>((uint32_t)(v >> 32)) << 32);
+    v ^= (uint64_t)dpp_u32<0x4E>((uint32_t)v) | ((uint64_t)dpp_u32<0x4E>((uint32_t)(v >> 32)) << 32);
+    v ^= (uint64_t)dpp_u32<0x141>((uint32_t)v) | ((uint64_t)dpp_u32<0x141>((uint32_t)(v >> 32)) << 32);
+    v ^= (uint64_t)dpp_u32<0x140>((uint32_t)v) | ((uint64_t)dpp_u32<0x140>((uint32_t)(v >> 32)) << 32);
+    return mix64(v ^ 0xCBF29CE484222325ull);
+}
+
+// The wave's due bins (first pass), four at a time. Preconditions (checked by
+// the caller): no coded source, n_pad = 64, P <= 128, unit delays with R <= 4
+// staged offset rows (or general delays), every due bin written.
+__device__ __forceinline__ void detect_quads(const qsft_peel_desc& d, int base, uint64_t todo, uint32_t nheld, QuadWaveLds& W,
+                             const double2* tab, const int8_t* offbuf, const int8_t* mtbuf, bool mt_staged) {
+    constexpr int it = 0;
+    const int lane = threadIdx.x & 63, g = lane >> 4, l16 = lane & 15;
+    const int q = d.q, n = d.n, b = d.b, P = d.P, Q = d.Q, n_pad = d.n_pad;
+    const int nw = n_pad >> 2;                                     // 16
+    const ModQ mq((uint32_t)q);
+    const double thr = d.cutoff * (double)P;
+    const double inv_p = 1.0 / (double)P;
+    const double2* __restrict__ Ub = reinterpret_cast<const double2*>(d.U);
+    const int CQ = d.C * Q;
+    QSFT_GLOBAL int32_t* __restrict__ mark = gptr(d.mark);
+    QSFT_GLOBAL int8_t* __restrict__ kcur = gptr(d.kbin);         // pass 0: slot (it & 1) = 0
+    QSFT_GLOBAL double* __restrict__ rcur = gptr(d.rho_bin);
+    const bool unit = (d.flags & QSFT_PEEL_UNIT_D) != 0;
+    const bool neg = (d.flags & QSFT_PEEL_UNIT_D_NEG) != 0;
+    const int p1 = d.p1, R = P / p1;
+    const uint32_t m1 = (uint32_t)(0xFFFFFFFFull / (uint32_t)p1) + 1u;   // p / p1 exact for p < 256
+    int wdig = 0;                                                  // q^(b - 1 - l16): weight of bin digit l16
+    if (l16 < b) {
+        wdig = 1;
+        for (int e = l16 + 1; e < b; ++e) wdig *= q;
+    }
+    const uint32_t scanned = (uint32_t)__popcll(todo);
+    int nz = (int)nheld, ns = 0;                                   // wave-uniform
+    const int nrow = (P + 63) >> 6;
+    const uint32_t rowoff = (uint32_t)min(lane, P - 1) * 16u;
+    // the next up to four due bins (wave-uniform), -1 past the end
+    auto take = [&](uint64_t& m, int (&tb)[kQuad]) {
+        int cnt = 0;
+#pragma unroll
+        for (int k = 0; k < kQuad; ++k) {
+            tb[k] = -1;
+            if (m) {
+                tb[k] = __builtin_ctzll(m);
+                m &= m - 1;
+                ++cnt;
+            }
+        }
+        return cnt;
+    };
+    auto issue = [&](const int (&tb)[kQuad], int slot) {
+#pragma unroll
+        for (int k = 0; k < kQuad; ++k) {
+            if (tb[k] < 0) continue;
+            const double2* cb = Ub + (size_t)(base + tb[k]) * P;
+            for (int r = 0; r < nrow; ++r)
+                dma16(cb, min(64 * r * 16u + rowoff, (uint32_t)(P - 1) * 16u),
+                      __builtin_amdgcn_readfirstlane(lds_addr(&W.col[slot][k][64 * r])));
+        }
+    };
+    uint64_t rest = todo;
+    int cb[kQuad];
+    take(rest, cb);
+    issue(cb, 0);
+    wait_vm<0>();
+    __asm__ volatile("" ::: "memory");
+    int slot = 0;
+    const int cbase = base / Q;
+    for (;;) {
+        int nb[kQuad];
+        const int ncnt = take(rest, nb);
+        if (ncnt) issue(nb, slot ^ 1);
+        const int my = g == 0 ? cb[0] : g == 1 ? cb[1] : g == 2 ? cb[2] : cb[3];
+        const bool active = my >= 0;
+        const int t = base + (active ? my : 0);
+        int c = cbase;
+        while (t >= (c + 1) * Q) ++c;
+        const int j = t - c * Q;
+        const double2* cw = W.col[slot][g];
+        double2 u[kQRows];
+#pragma unroll
+        for (int r = 0; r < kQRows; ++r) {
+            const int p = l16 + 16 * r;
+            u[r] = (active && p < P) ? cw[p] : make_double2(0.0, 0.0);
+        }
+        // |col|^2: bin_energy's per-lane sums of the one-bin layout (rows L, L + 64)
+        double es[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            double sv = 0.0;
+            sv = __fma_rn(u[v].x, u[v].x, sv);
+            sv = __fma_rn(u[v].y, u[v].y, sv);
+            sv = __fma_rn(u[v + 4].x, u[v + 4].x, sv);
+            sv = __fma_rn(u[v + 4].y, u[v + 4].y, sv);
+            es[v] = sv;
+        }
+        const double energy = quad_total(es);
+        const bool nonzero = active && energy > thr;               // uniform over the bin's 16 lanes
+        nz += (int)__popcll(__ballot(l16 == 0 && nonzero));
+        W.kws[g][l16] = 0u;
+        wave_lds_sync();
+        // --- k: coordinates i = 1 + l16 + 16 m of the bin
+        int8_t* kb8 = reinterpret_cast<int8_t*>(W.kws[g]);
+        if (nonzero) {
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                const int i = 1 + l16 + 16 * m;
+                if (i <= n) kb8[i - 1] = (int8_t)coord_digit(d, cw, i, q, P);
+            }
+        }
+        wave_lds_sync();
+        // --- signature phases, rho, residual
+        const uint32_t* kk = W.kws[g];
+        int o[4] = {0, 0, 0, 0};
+        if (unit) {
+            if (nonzero && l16 < R) W.off[g][l16] = (int)mq(dot_row_lds(offbuf + (size_t)l16 * n_pad, kk, nw));
+            wave_lds_sync();
+            const int4 ov = *reinterpret_cast<const int4*>(W.off[g]);
+            o[0] = ov.x;
+            o[1] = ov.y;
+            o[2] = ov.z;
+            o[3] = ov.w;
+        }
+        const QSFT_GLOBAL int8_t* __restrict__ Dc =
+            gptr(d.D) + (size_t)((d.flags & QSFT_PEEL_SHARED_D) ? 0 : c) * P * n_pad;
+        double2 w[kQRows];
+#pragma unroll
+        for (int r = 0; r < kQRows; ++r) {
+            const int p = l16 + 16 * r;
+            w[r] = make_double2(0.0, 0.0);
+            if (nonzero && p < P) {
+                if (unit) {
+                    const int blk = (int)__umulhi((uint32_t)p, m1);
+                    const int i = p - blk * p1;
+                    int e = blk == 0 ? o[0] : blk == 1 ? o[1] : blk == 2 ? o[2] : o[3];
+                    if (i > 0) e += neg ? q - (int)kb8[i - 1] : (int)kb8[i - 1];
+                    if (e >= q) e -= q;
+                    w[r] = tab[e];
+                } else {
+                    w[r] = tab[mq(dot_row(Dc + (size_t)p * n_pad, kk, nw))];
+                }
+            }
+        }
+        double rs[4], is[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            double rr = 0.0, ri = 0.0;
+            if (l16 + 16 * v < P) conj_dot_acc(w[v], u[v], rr, ri);
+            if (l16 + 16 * v + 64 < P) conj_dot_acc(w[v + 4], u[v + 4], rr, ri);
+            rs[v] = rr;
+            is[v] = ri;
+        }
+        const double rho_x = quad_total(rs) * inv_p, rho_y = quad_total(is) * inv_p;
+        double res = residual_energy(energy, P, rho_x, rho_y);
+        if (__ballot(nonzero && energy > 1e12 * thr)) {          // explicit residual (as detect_bins)
+            double ss[4];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                double sv = 0.0;
+                if (l16 + 16 * v < P) residual_acc(u[v], w[v], rho_x, rho_y, sv);
+                if (l16 + 16 * v + 64 < P) residual_acc(u[v + 4], w[v + 4], rho_x, rho_y, sv);
+                ss[v] = sv;
+            }
+            const double sres = quad_total(ss);
+            if (energy > 1e12 * thr) res = sres;
+        }
+        // --- bin match: sum over digits dg (lane dg of the bin) of (<M_c^T row dg, k> mod q) q^(b - 1 - dg)
+        int hd = 0;
+        if (nonzero && l16 < b) {
+            const size_t mo = ((size_t)c * b + l16) * n_pad;
+            hd = (int)mq(mt_staged ? dot_row_lds(mtbuf + mo, kk, nw) : dot_row(gptr(d.Mt) + mo, kk, nw)) * wdig;
+        }
+        const bool match = row_sum_i(hd) == j;
+        const bool single = nonzero && match && res <= thr;
+        const int mval = nonzero ? (single ? it + 1 : (it + 1) | kHeldMulti) : 0;
+        const uint64_t hk = hash_k16(kk[l16], l16);
+        const uint64_t smask = __ballot(l16 == 0 && single);
+        if (l16 == 0 && single) {
+            const int at = ns + (int)__popcll(smask & ((1ull << lane) - 1ull));
+            W.sb_t[at] = t;
+            W.sb_h[at] = hk;
+            W.sb_rho[at] = make_double2(rho_x, rho_y);
+        }
+        ns += (int)__popcll(smask);
+        wait_vm<0>();                                              // the next quad's columns have landed
+        __asm__ volatile("" ::: "memory");
+        if (mval && l16 == 0) mark[t] = mval;
+        if (single) {
+            reinterpret_cast<QSFT_GLOBAL uint32_t*>(kcur + (size_t)t * n_pad)[l16] = kk[l16];
+            if (l16 == 0) {
+                rcur[2 * (size_t)t] = rho_x;
+                rcur[2 * (size_t)t + 1] = rho_y;
+            }
+        }
+        if (!ncnt) break;
+#pragma unroll
+        for (int k = 0; k < kQuad; ++k) cb[k] = nb[k];
+        slot ^= 1;
+        wave_lds_sync();                                           // kws / this slot are rewritten next
+    }
+    (void)CQ;
+    finish_wave(d, it, scanned, nz, ns, W.sb_t, W.sb_h, W.sb_rho, lane);
+}
+
+// Kernel A of the first pass: detect_quads where its preconditions hold (and
+// QSFT_PEEL_ONE_BIN_WAVES is not set), else detect_bins. Same grid and bin
+// partition as k_peel_detect. Its wave LDS (18 KiB) allows two blocks per CU,
+// each wave with four bins in flight.
+__global__ __launch_bounds__(kBinThreads) __attribute__((amdgpu_waves_per_eu(2))) void k_peel_detect0(
+    const qsft_peel_desc* __restrict__ descs, const int32_t* __restrict__ pre, int n_inst) {
+    __shared__ double2 tab[QSFT_MAX_Q];
+    __shared__ union WaveLds {
+        QuadWaveLds quad;
+        DetectWaveLds<2> one;
+    } wl[kWaves];
+    __shared__ __align__(16) int8_t offbuf[kStageOffBytes];
+    __shared__ __align__(16) int8_t mtbuf[kStageMtBytes];
+    const int inst = find_inst(pre, n_inst, blockIdx.x);
+    const qsft_peel_desc& d = descs[inst];
+    uint32_t* __restrict__ ctr = d.ctr;
+    constexpr int it = 0;
+    if (it >= d.max_iter) return;
+    if (cptr(ctr)[it * QSFT_CTR_ITER_STRIDE + 3] == 0) return;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int base = ((blockIdx.x - pre[inst]) * kWaves + wave) * kBinsPerWave;
+    uint32_t nheld;
+    const uint64_t todo = scan_marks(d, it, base, nheld);
+    if (!__syncthreads_or(todo != 0)) {
+        if (lane == 0 && nheld) atomicAdd(ctr + it * QSFT_CTR_ITER_STRIDE + 1, nheld);
+        return;
+    }
+    fill_wtable_d(tab, d.q);
+    const bool offs_staged = stage_offsets(d, offbuf);
+    const bool mt_staged = stage_mt(d, mtbuf);
+    __syncthreads();
+    const bool unit = (d.flags & QSFT_PEEL_UNIT_D) != 0;
+    // every due bin of the wave written by the assembler (sparse U)
+    const bool written = !sparse_u(d) || __ballot(((todo >> lane) & 1) && !gptr(d.written)[base + lane]) == 0;
+    const bool quads = !(d.flags & QSFT_PEEL_ONE_BIN_WAVES) && !(d.method & QSFT_DETECT_CODED) && d.n_pad == 64 &&
+                       d.P <= 128 && (!unit || (offs_staged && d.P / d.p1 <= 4)) && written;
+    if (quads) detect_quads(d, base, todo, nheld, wl[wave].quad, tab, offbuf, mtbuf, mt_staged);
+    else detect_bins<2, false>(d, it, base, todo, nheld, wl[wave].one, tab, offbuf, offs_staged, mtbuf, mt_staged);
+}
 
 // Kernel A. Wave w of an instance owns bins [64 w, 64 w + 64): it classifies
 // the ones this pass scans (mark == pass), prefetching
@@ -1691,8 +2004,11 @@ extern "C" int qsft_peel_iteration(const qsft_peel_desc* d, const int32_t* pre_b
         return QSFT_ERR_ARG;
     }
     hipStream_t st = (hipStream_t)stream;
-    // rows per lane: 2 when every instance has P <= 128, else 4
-    if (max_p <= 128)
+    // rows per lane: 2 when every instance has P <= 128, else 4; the first
+    // pass of the former four bins per wave where it can
+    if (max_p <= 128 && it == 0)
+        hipLaunchKernelGGL(k_peel_detect0, dim3(blocks_bins), dim3(kBinThreads), 0, st, d, pre_bins, n_inst);
+    else if (max_p <= 128)
         hipLaunchKernelGGL(k_peel_detect<2>, dim3(blocks_bins), dim3(kBinThreads), 0, st, d, pre_bins, n_inst, it);
     else
         hipLaunchKernelGGL(k_peel_detect<4>, dim3(blocks_bins), dim3(kBinThreads), 0, st, d, pre_bins, n_inst, it);

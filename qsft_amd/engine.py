@@ -135,10 +135,12 @@ class Batch:
     FP32 samples): the assembler writes only the columns above the zeroton
     cutoff; the peeler rebuilds another column from the samples and the same
     noise when a ball first lands on it (about a fifth of U is written at the
-    K1 shape)."""
+    K1 shape). one_bin_waves: the first detection pass one bin per wave (the
+    default classifies four bins per wave where it can; same results)."""
 
     def __init__(self, specs, precision="fp32", max_iter=15, device=None, eval_mode=None, fused_screen=True,
-                 host_rows=True, tail_stream=True, own_stream=False, sparse_u=True, fuse_from=3):
+                 host_rows=True, tail_stream=True, own_stream=False, sparse_u=True, fuse_from=3,
+                 one_bin_waves=False):
         self.specs = list(specs)
         self.precision = precision
         if eval_mode is None:
@@ -175,7 +177,8 @@ class Batch:
             asm.append(item)
             pl.append(dict(q=sp.q, n=sp.n, b=sp.b, C=C, P=P, p1=sp.n + 1,
                            method=L.QSFT_DETECT_NSO1 if sp.method == "nso" else L.QSFT_DETECT_NOISELESS,
-                           cutoff=sp.threshold(), Mt=Mt, D=Dd, U=U, shared_d=True, energy=energy))
+                           cutoff=sp.threshold(), Mt=Mt, D=Dd, U=U, shared_d=True, energy=energy,
+                           one_bin_waves=one_bin_waves))
             if sparse_u and energy is not None and precision == "fp32":
                 pl[-1]["sparse"] = dict(x=x, sigma=sigma)
         self.eval = K.EvalMfma(ev) if eval_mode == "mfma" else K.EvalCosets(ev, code)

@@ -447,7 +447,7 @@ def unit_delay_sign(D, q, n, p1):
 
 
 def peel_flags(it):
-    """QSFT_PEEL_SHARED_D / UNIT_D / UNIT_D_NEG bits of a peel item (see PeelBatch)."""
+    """QSFT_PEEL_SHARED_D / UNIT_D / UNIT_D_NEG (/ ONE_BIN_WAVES) bits of a peel item (see PeelBatch)."""
     C, q, n = it["C"], it["q"], it["n"]
     shared = it.get("shared_d")
     if shared is None:
@@ -459,6 +459,8 @@ def peel_flags(it):
         unit = unit_delay_sign(it["D"][:1] if shared else it["D"], q, n, it["p1"])
     if unit:
         flags |= L.QSFT_PEEL_UNIT_D | (L.QSFT_PEEL_UNIT_D_NEG if unit < 0 else 0)
+    if it.get("one_bin_waves"):
+        flags |= L.QSFT_PEEL_ONE_BIN_WAVES
     return flags
 
 

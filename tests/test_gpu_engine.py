@@ -182,6 +182,39 @@ def test_fused_tail_passes_match_per_pass_launches(case):
             assert np.array_equal(a[2], b[2])                         # detection order
 
 
+@pytest.mark.parametrize("case", ["k1_mixed_snr", "tiny_and_odd_q", "k2_q20", "everything_rebuilt"])
+def test_first_pass_four_bins_per_wave_matches_one_bin_per_wave(case):
+    """The first detection pass classifies four bins per wave (16 lanes each,
+    the one-bin layout's per-lane sums rebuilt lane by lane): bit for bit the
+    rows, order and pass count of the one-bin-per-wave pass, for noisy and
+    noiseless K1 transforms, groups smaller than a wave (q^b < 64), q = 20,
+    and a first pass whose columns are all rebuilt (the one-bin path then
+    runs, the quad preconditions failing)."""
+    rng = np.random.default_rng(47)
+    if case == "k1_mixed_snr":
+        specs = [E.random_spec(4, 50, 1000, 6, 3, 2, snr, rng=rng) for snr in (0.0, 5.0, 10.0, 10.0, 20.0, None)]
+    elif case == "tiny_and_odd_q":
+        specs = [E.random_spec(2, 8, 6, 3, 3, 2, None, rng=rng), E.random_spec(2, 12, 10, 3, 3, 2, 10.0, rng=rng),
+                 E.random_spec(3, 10, 40, 4, 3, 2, 15.0, rng=rng), E.random_spec(4, 20, 50, 2, 3, 1, 12.0, rng=rng)]
+    elif case == "k2_q20":
+        specs = [E.random_spec(20, 16, 2000, 3, 3, 2, 20.0, rng=rng) for _ in range(3)]
+    else:
+        specs = [E.random_spec(4, 50, 1000, 6, 3, 2, 10.0, rng=rng), E.random_spec(4, 30, 100, 4, 3, 2, None, rng=rng)]
+    print("four bins per wave vs one,", case, flush=True)
+    runs = []
+    for one in (False, True):
+        bt = E.Batch(specs, precision="fp32", one_bin_waves=one)
+        if case == "everything_rebuilt":
+            bt.asm = K.Assemble([dict(it, cutoff=float("inf")) for it in bt.asm_items])
+        bt.launch(seed_offset=9)
+        runs.append((bt.results(), bt.peel.last_iters))
+    (a_res, a_it), (b_res, b_it) = runs
+    assert a_it == b_it
+    for a, b in zip(a_res, b_res):
+        _identical(a, b)
+        assert np.array_equal(a[2], b[2])                             # detection order
+
+
 def _written_flags(bt, i):
     d, ws = bt.peel.descs[i], bt.peel.ws[i]
     off = d.written - ws.data_ptr()
