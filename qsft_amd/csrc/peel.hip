@@ -1621,9 +1621,8 @@ __global__ __launch_bounds__(64 * tail_waves<NR>()) void k_peel_tail(const qsft_
     }
 }
 
-// Output of one key: the running mean of its records in time order. Up to
-// kFinRecs records are gathered into registers by one walk of the key's list
-// and ordered there; keys with more records walk the list once per record.
+// Output of one key: the running mean of its records in time order, gathered
+// kFinRecs at a time by walks of the key's list.
 constexpr int kFinRecs = 8;
 
 __global__ __launch_bounds__(kSlotThreads) void k_peel_finalize(const qsft_peel_desc* __restrict__ descs,
@@ -1654,53 +1653,49 @@ __global__ __launch_bounds__(kSlotThreads) void k_peel_finalize(const qsft_peel_
             gy = __ddiv_rn(__dadd_rn(__dmul_rn(gy, cd), vy), cd + 1.0);
         }
     };
-    if (m <= (uint32_t)kFinRecs) {
+    // Each walk of the list keeps, in registers and sorted by time, the
+    // kFinRecs earliest records later than the last one accumulated, then
+    // accumulates them in order: ceil(m / kFinRecs) walks (one for most keys).
+    uint64_t prev = 0;                                             // record times are > 0
+    uint32_t done = 0;
+    while (done < m) {
         uint64_t tr[kFinRecs];
         double rx[kFinRecs], ry[kFinRecs];
-        uint32_t got = 0;
-        uint32_t r = d.head[h];
 #pragma unroll
         for (int a = 0; a < kFinRecs; ++a) {
             tr[a] = ~0ull;
             rx[a] = ry[a] = 0.0;
-            if ((uint32_t)a < m && r != kNil) {
-                tr[a] = d.rec_time[r];
-                rx[a] = d.rec_rho[2 * (size_t)r];
-                ry[a] = d.rec_rho[2 * (size_t)r + 1];
-                r = d.rec_next[r];
+        }
+        uint32_t steps = 0;
+        for (uint32_t r = d.head[h]; r != kNil && steps <= L; r = d.rec_next[r], ++steps) {
+            uint64_t t = d.rec_time[r];
+            if (t <= prev || t >= tr[kFinRecs - 1]) continue;
+            double vx = d.rec_rho[2 * (size_t)r], vy = d.rec_rho[2 * (size_t)r + 1];
+#pragma unroll
+            for (int a = 0; a < kFinRecs; ++a) {                   // insertion: the later entries shift up
+                if (t < tr[a]) {
+                    const uint64_t tt = tr[a];
+                    const double xx = rx[a], yy = ry[a];
+                    tr[a] = t;
+                    rx[a] = vx;
+                    ry[a] = vy;
+                    t = tt;
+                    vx = xx;
+                    vy = yy;
+                }
+            }
+        }
+        uint32_t got = 0;
+#pragma unroll
+        for (int a = 0; a < kFinRecs; ++a) {
+            if (tr[a] != ~0ull && done < m) {
+                accumulate(done, rx[a], ry[a]);
+                prev = tr[a];
+                ++done;
                 ++got;
             }
         }
-        // times are distinct (one record per pass and bin): rank a = position of record a
-        uint64_t prev = 0;
-        for (uint32_t cnt = 0; cnt < got; ++cnt) {
-            uint64_t best = ~0ull;
-            double vx = 0.0, vy = 0.0;
-#pragma unroll
-            for (int a = 0; a < kFinRecs; ++a) {
-                if (tr[a] > prev && tr[a] < best) {
-                    best = tr[a];
-                    vx = rx[a];
-                    vy = ry[a];
-                }
-            }
-            accumulate(cnt, vx, vy);
-            prev = best;
-        }
-    } else {
-        uint64_t prev = 0;
-        for (uint32_t cnt = 0; cnt < m; ++cnt) {
-            uint64_t best = ~0ull;
-            uint32_t bi = kNil;
-            uint32_t steps = 0;
-            for (uint32_t r = d.head[h]; r != kNil && steps <= L; r = d.rec_next[r], ++steps) {
-                const uint64_t tmr = d.rec_time[r];
-                if (tmr > prev && tmr < best) { best = tmr; bi = r; }
-            }
-            if (bi == kNil) break;
-            accumulate(cnt, d.rec_rho[2 * (size_t)bi], d.rec_rho[2 * (size_t)bi + 1]);
-            prev = best;
-        }
+        if (got == 0) break;                                       // (a list shorter than its count)
     }
     const int nw = d.n_pad >> 2;
     const uint32_t* ksrc = reinterpret_cast<const uint32_t*>(d.key + (size_t)h * d.n_pad);
