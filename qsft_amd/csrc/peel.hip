@@ -102,12 +102,14 @@ __device__ __forceinline__ uint32_t dot_row_lds(const int8_t* row, const uint32_
 // Block-level LDS copies of small per-instance tables (filled once per block):
 // the offset rows of structured delays (group 0 when shared) and every
 // group's M^T rows. Larger tables stay in global memory.
+// MAX: the buffer's size (the first-pass kernel has smaller ones).
 constexpr int kStageOffBytes = 2048;
 constexpr int kStageMtBytes = 4096;
 
+template <int MAX = kStageOffBytes>
 __device__ __forceinline__ bool stage_offsets(const qsft_peel_desc& d, int8_t* buf) {
     const int R = d.P / d.p1, n_pad = d.n_pad;
-    if (!(d.flags & QSFT_PEEL_UNIT_D) || !(d.flags & QSFT_PEEL_SHARED_D) || R * n_pad > kStageOffBytes)
+    if (!(d.flags & QSFT_PEEL_UNIT_D) || !(d.flags & QSFT_PEEL_SHARED_D) || R * n_pad > MAX)
         return false;
     const int words = R * n_pad / 16;
     for (int e = threadIdx.x; e < words; e += blockDim.x) {
@@ -117,9 +119,10 @@ __device__ __forceinline__ bool stage_offsets(const qsft_peel_desc& d, int8_t* b
     return true;
 }
 
+template <int MAX = kStageMtBytes>
 __device__ __forceinline__ bool stage_mt(const qsft_peel_desc& d, int8_t* buf) {
     const int bytes = d.C * d.b * d.n_pad;
-    if (bytes > kStageMtBytes) return false;
+    if (bytes > MAX) return false;
     for (int e = threadIdx.x; e < bytes / 16; e += blockDim.x)
         reinterpret_cast<gu4*>(buf)[e] = reinterpret_cast<const QSFT_GLOBAL gu4*>(gptr(d.Mt))[e];
     return true;
@@ -775,16 +778,22 @@ __device__ __noinline__ void rs_decode_wave(const qsft_peel_desc& d, const int8_
     }
 }
 
-template <int NR>
+// Singletons a wave buffers in LDS before registering them (CAP of them; a
+// full buffer is registered mid-loop), so that the buffers stay small enough
+// for four detection blocks per CU. The fused tail, one workgroup per CU
+// anyway, buffers all 64.
+constexpr int kSbCap = 16;
+
+template <int NR, int CAP = kSbCap>
 struct DetectWaveLds {
     __align__(16) uint32_t kws[QSFT_MAX_N / 4];
     int8_t syn[kRsMaxSyn];                                         // coded source: syndrome digits
     int32_t rs[kRsScratch];                                        // coded source: decoder state
     __align__(16) PendingSlot pslot[2];                            // first queued ball, per column slot
     __align__(16) double2 colw[2][64 * NR];                        // current / next column
-    uint64_t sb_h[kBinsPerWave];
-    double2 sb_rho[kBinsPerWave];
-    int32_t sb_t[kBinsPerWave];
+    uint64_t sb_h[CAP];
+    double2 sb_rho[CAP];
+    int32_t sb_t[CAP];
 };
 
 // The wave's bins due this pass among the 64 from `base` (bit i: bin base + i).
@@ -809,20 +818,12 @@ __device__ __forceinline__ uint64_t scan_marks(const qsft_peel_desc& d, int it, 
 // multitons: one atomic each per wave), then the hash registration of the
 // wave's buffered singletons, one lane each, so the atomics of up to 64
 // registrations are in flight together.
-__device__ __forceinline__ void finish_wave(const qsft_peel_desc& d, int it, uint32_t scanned, int nz, int ns,
-                                            const int32_t* sb_t, const uint64_t* sb_h, const double2* sb_rho,
-                                            int lane) {
+// Hash registration of the wave's ns buffered singletons (sb_*), one lane
+// each. Every lane of the wave takes part (ballots).
+__device__ __forceinline__ void register_singletons(const qsft_peel_desc& d, int it, int ns, const int32_t* sb_t,
+                                                    const uint64_t* sb_h, const double2* sb_rho, int lane) {
     uint32_t* __restrict__ ctr = d.ctr;
     const int CQ = d.C * d.Q;
-    if (lane == 0) {
-        uint32_t* pc = ctr + it * QSFT_CTR_ITER_STRIDE;
-        atomicAdd(pc + 4, scanned);
-        if (ns) atomicAdd(pc + 0, (uint32_t)ns);
-        if (nz > ns) atomicAdd(pc + 1, (uint32_t)(nz - ns));
-    }
-    if (ns == 0) return;
-
-    // --- hash registration, one lane per singleton of this wave
     int slot = -1;
     bool claimed = false;                                      // this lane's k took a new slot
     bool newball = false;
@@ -882,9 +883,24 @@ __device__ __forceinline__ void finish_wave(const qsft_peel_desc& d, int it, uin
     }
 }
 
-template <int NR, bool PERSIST>
+// End of a wave's detection: the pass counters over the whole wave (scanned
+// bins, ns singletons, nz - ns multitons) and the registration of the nbuf
+// singletons still buffered.
+__device__ __forceinline__ void finish_wave(const qsft_peel_desc& d, int it, uint32_t scanned, int nz, int ns,
+                                            const int32_t* sb_t, const uint64_t* sb_h, const double2* sb_rho,
+                                            int lane, int nbuf) {
+    if (lane == 0) {
+        uint32_t* pc = d.ctr + it * QSFT_CTR_ITER_STRIDE;
+        atomicAdd(pc + 4, scanned);
+        if (ns) atomicAdd(pc + 0, (uint32_t)ns);
+        if (nz > ns) atomicAdd(pc + 1, (uint32_t)(nz - ns));
+    }
+    if (nbuf > 0) register_singletons(d, it, nbuf, sb_t, sb_h, sb_rho, lane);
+}
+
+template <int NR, bool PERSIST, int CAP>
 __device__ __forceinline__ void detect_bins(const qsft_peel_desc& d, int it, int base, uint64_t todo, uint32_t nheld,
-                                            DetectWaveLds<NR>& W, const double2* tab, const int8_t* offbuf,
+                                            DetectWaveLds<NR, CAP>& W, const double2* tab, const int8_t* offbuf,
                                             bool offs_staged, const int8_t* mtbuf, bool mt_staged) {
     uint32_t* __restrict__ ctr = d.ctr;
     const int lane = threadIdx.x & 63;
@@ -911,6 +927,7 @@ __device__ __forceinline__ void detect_bins(const qsft_peel_desc& d, int it, int
     }
     const uint32_t scanned = (uint32_t)__popcll(todo);
     int nz = (int)nheld, ns = 0;                                   // wave-uniform
+    int nb = 0;                                                    // singletons buffered in W.sb_*
     // sparse U: due bins whose column the assembler skipped (only this wave
     // writes its bins' flags, in store_col, after their step)
     const uint64_t unwritten =
@@ -1050,11 +1067,12 @@ __device__ __forceinline__ void detect_bins(const qsft_peel_desc& d, int it, int
             if (single) {
                 const uint64_t hk = hash_k(kk, nw, lane);
                 if (lane == 0) {
-                    W.sb_t[ns] = t;
-                    W.sb_h[ns] = hk;
-                    W.sb_rho[ns] = make_double2(rho_x, rho_y);
+                    W.sb_t[nb] = t;
+                    W.sb_h[nb] = hk;
+                    W.sb_rho[nb] = make_double2(rho_x, rho_y);
                 }
                 ++ns;
+                ++nb;
             }
         }
         wait_vm<0>();                                              // the next bin's data has landed
@@ -1074,9 +1092,16 @@ __device__ __forceinline__ void detect_bins(const qsft_peel_desc& d, int it, int
         curent = nent;
         cslot ^= 1;
         wave_lds_sync();                                           // kk / this slot are rewritten next
+        if constexpr (CAP < kBinsPerWave) {
+            if (nb == CAP) {                                       // the buffer is full: register it
+                register_singletons(d, it, nb, W.sb_t, W.sb_h, W.sb_rho, lane);
+                nb = 0;
+                wave_lds_sync();
+            }
+        }
     }
 
-    finish_wave(d, it, scanned, nz, ns, W.sb_t, W.sb_h, W.sb_rho, lane);
+    finish_wave(d, it, scanned, nz, ns, W.sb_t, W.sb_h, W.sb_rho, lane, nb);
 }
 
 // ---- The first pass, four bins per wave ------------------------------------
@@ -1096,12 +1121,12 @@ constexpr int kQuad = 4;                 // bins a wave classifies at once
 constexpr int kQRows = 8;                // rows per lane (P <= 128)
 
 struct QuadWaveLds {
-    __align__(16) double2 col[2][kQuad][128];      // current / next quad's columns
+    __align__(16) double2 col[kQuad][128];         // the current quad's columns
     __align__(16) uint32_t kws[kQuad][16];         // each bin's k (n_pad = 64)
     __align__(16) int32_t off[kQuad][4];           // each bin's <offset_r, k> mod q (unit delays)
-    uint64_t sb_h[kBinsPerWave];
-    double2 sb_rho[kBinsPerWave];
-    int32_t sb_t[kBinsPerWave];
+    uint64_t sb_h[kSbCap];
+    double2 sb_rho[kSbCap];
+    int32_t sb_t[kSbCap];
 };
 
 // In-row (16 lanes) part of wave_sum: the same DPP butterfly.
@@ -1155,6 +1180,7 @@ __device__ __forceinline__ void detect_quads(const qsft_peel_desc& d, int base, 
     }
     const uint32_t scanned = (uint32_t)__popcll(todo);
     int nz = (int)nheld, ns = 0;                                   // wave-uniform
+    int nbuf = 0;                                                  // singletons buffered in W.sb_*
     const int nrow = (P + 63) >> 6;
     const uint32_t rowoff = (uint32_t)min(lane, P - 1) * 16u;
     // the next up to four due bins (wave-uniform), -1 past the end
@@ -1171,50 +1197,51 @@ __device__ __forceinline__ void detect_quads(const qsft_peel_desc& d, int base, 
         }
         return cnt;
     };
-    auto issue = [&](const int (&tb)[kQuad], int slot) {
+    auto issue = [&](const int (&tb)[kQuad]) {
 #pragma unroll
         for (int k = 0; k < kQuad; ++k) {
             if (tb[k] < 0) continue;
             const double2* cb = Ub + (size_t)(base + tb[k]) * P;
             for (int r = 0; r < nrow; ++r)
                 dma16(cb, min(64 * r * 16u + rowoff, (uint32_t)(P - 1) * 16u),
-                      __builtin_amdgcn_readfirstlane(lds_addr(&W.col[slot][k][64 * r])));
+                      __builtin_amdgcn_readfirstlane(lds_addr(&W.col[k][64 * r])));
         }
     };
     uint64_t rest = todo;
     int cb[kQuad];
     take(rest, cb);
-    issue(cb, 0);
-    wait_vm<0>();
-    __asm__ volatile("" ::: "memory");
-    int slot = 0;
     const int cbase = base / Q;
     for (;;) {
+        // the quad's columns into LDS (one buffer: the other waves of the
+        // SIMD hide the load)
+        issue(cb);
+        wait_vm<0>();
+        __asm__ volatile("" ::: "memory");
         int nb[kQuad];
         const int ncnt = take(rest, nb);
-        if (ncnt) issue(nb, slot ^ 1);
         const int my = g == 0 ? cb[0] : g == 1 ? cb[1] : g == 2 ? cb[2] : cb[3];
         const bool active = my >= 0;
         const int t = base + (active ? my : 0);
         int c = cbase;
         while (t >= (c + 1) * Q) ++c;
         const int j = t - c * Q;
-        const double2* cw = W.col[slot][g];
-        double2 u[kQRows];
-#pragma unroll
-        for (int r = 0; r < kQRows; ++r) {
+        const double2* cw = W.col[g];
+        // row r of the lane (p = l16 + 16 r) from the LDS column, read where it
+        // is used so that no column is held in registers across the k digits
+        auto row = [&](int r) {
             const int p = l16 + 16 * r;
-            u[r] = (active && p < P) ? cw[p] : make_double2(0.0, 0.0);
-        }
+            return (active && p < P) ? cw[p] : make_double2(0.0, 0.0);
+        };
         // |col|^2: bin_energy's per-lane sums of the one-bin layout (rows L, L + 64)
         double es[4];
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
+            const double2 a = row(v), e = row(v + 4);
             double sv = 0.0;
-            sv = __fma_rn(u[v].x, u[v].x, sv);
-            sv = __fma_rn(u[v].y, u[v].y, sv);
-            sv = __fma_rn(u[v + 4].x, u[v + 4].x, sv);
-            sv = __fma_rn(u[v + 4].y, u[v + 4].y, sv);
+            sv = __fma_rn(a.x, a.x, sv);
+            sv = __fma_rn(a.y, a.y, sv);
+            sv = __fma_rn(e.x, e.x, sv);
+            sv = __fma_rn(e.y, e.y, sv);
             es[v] = sv;
         }
         const double energy = quad_total(es);
@@ -1246,11 +1273,10 @@ __device__ __forceinline__ void detect_quads(const qsft_peel_desc& d, int base, 
         }
         const QSFT_GLOBAL int8_t* __restrict__ Dc =
             gptr(d.D) + (size_t)((d.flags & QSFT_PEEL_SHARED_D) ? 0 : c) * P * n_pad;
-        double2 w[kQRows];
-#pragma unroll
-        for (int r = 0; r < kQRows; ++r) {
+        // signature phase of row r of the lane, formed where it is used
+        auto phase = [&](int r) {
             const int p = l16 + 16 * r;
-            w[r] = make_double2(0.0, 0.0);
+            double2 w = make_double2(0.0, 0.0);
             if (nonzero && p < P) {
                 if (unit) {
                     const int blk = (int)__umulhi((uint32_t)p, m1);
@@ -1258,18 +1284,19 @@ __device__ __forceinline__ void detect_quads(const qsft_peel_desc& d, int base, 
                     int e = blk == 0 ? o[0] : blk == 1 ? o[1] : blk == 2 ? o[2] : o[3];
                     if (i > 0) e += neg ? q - (int)kb8[i - 1] : (int)kb8[i - 1];
                     if (e >= q) e -= q;
-                    w[r] = tab[e];
+                    w = tab[e];
                 } else {
-                    w[r] = tab[mq(dot_row(Dc + (size_t)p * n_pad, kk, nw))];
+                    w = tab[mq(dot_row(Dc + (size_t)p * n_pad, kk, nw))];
                 }
             }
-        }
+            return w;
+        };
         double rs[4], is[4];
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             double rr = 0.0, ri = 0.0;
-            if (l16 + 16 * v < P) conj_dot_acc(w[v], u[v], rr, ri);
-            if (l16 + 16 * v + 64 < P) conj_dot_acc(w[v + 4], u[v + 4], rr, ri);
+            if (l16 + 16 * v < P) conj_dot_acc(phase(v), row(v), rr, ri);
+            if (l16 + 16 * v + 64 < P) conj_dot_acc(phase(v + 4), row(v + 4), rr, ri);
             rs[v] = rr;
             is[v] = ri;
         }
@@ -1280,8 +1307,8 @@ __device__ __forceinline__ void detect_quads(const qsft_peel_desc& d, int base, 
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 double sv = 0.0;
-                if (l16 + 16 * v < P) residual_acc(u[v], w[v], rho_x, rho_y, sv);
-                if (l16 + 16 * v + 64 < P) residual_acc(u[v + 4], w[v + 4], rho_x, rho_y, sv);
+                if (l16 + 16 * v < P) residual_acc(row(v), phase(v), rho_x, rho_y, sv);
+                if (l16 + 16 * v + 64 < P) residual_acc(row(v + 4), phase(v + 4), rho_x, rho_y, sv);
                 ss[v] = sv;
             }
             const double sres = quad_total(ss);
@@ -1298,15 +1325,20 @@ __device__ __forceinline__ void detect_quads(const qsft_peel_desc& d, int base, 
         const int mval = nonzero ? (single ? it + 1 : (it + 1) | kHeldMulti) : 0;
         const uint64_t hk = hash_k16(kk[l16], l16);
         const uint64_t smask = __ballot(l16 == 0 && single);
+        const int nnew = (int)__popcll(smask);
+        if (nbuf + nnew > kSbCap) {                                // no room: register the buffer first
+            register_singletons(d, it, nbuf, W.sb_t, W.sb_h, W.sb_rho, lane);
+            nbuf = 0;
+            wave_lds_sync();
+        }
         if (l16 == 0 && single) {
-            const int at = ns + (int)__popcll(smask & ((1ull << lane) - 1ull));
+            const int at = nbuf + (int)__popcll(smask & ((1ull << lane) - 1ull));
             W.sb_t[at] = t;
             W.sb_h[at] = hk;
             W.sb_rho[at] = make_double2(rho_x, rho_y);
         }
-        ns += (int)__popcll(smask);
-        wait_vm<0>();                                              // the next quad's columns have landed
-        __asm__ volatile("" ::: "memory");
+        ns += nnew;
+        nbuf += nnew;
         if (mval && l16 == 0) mark[t] = mval;
         if (single) {
             reinterpret_cast<QSFT_GLOBAL uint32_t*>(kcur + (size_t)t * n_pad)[l16] = kk[l16];
@@ -1318,26 +1350,32 @@ __device__ __forceinline__ void detect_quads(const qsft_peel_desc& d, int base, 
         if (!ncnt) break;
 #pragma unroll
         for (int k = 0; k < kQuad; ++k) cb[k] = nb[k];
-        slot ^= 1;
-        wave_lds_sync();                                           // kws / this slot are rewritten next
+        wave_lds_sync();                                           // kws / the column buffer are rewritten next
     }
     (void)CQ;
-    finish_wave(d, it, scanned, nz, ns, W.sb_t, W.sb_h, W.sb_rho, lane);
+    finish_wave(d, it, scanned, nz, ns, W.sb_t, W.sb_h, W.sb_rho, lane, nbuf);
 }
 
 // Kernel A of the first pass: detect_quads where its preconditions hold (and
 // QSFT_PEEL_ONE_BIN_WAVES is not set), else detect_bins. Same grid and bin
-// partition as k_peel_detect. Its wave LDS (18 KiB) allows two blocks per CU,
-// each wave with four bins in flight.
-__global__ __launch_bounds__(kBinThreads) __attribute__((amdgpu_waves_per_eu(2))) void k_peel_detect0(
+// partition as k_peel_detect. The block's LDS (one column buffer of four bins
+// per wave, 16-entry singleton buffers, half-size staging) is under 40 KiB, so
+// four blocks fit a CU: four waves per SIMD (128 VGPRs) with four bins each.
+// Measured on the K1 64-batch's first pass against the earlier forms: two
+// waves with two column buffers 0.24 ms, three waves with one 0.21 ms, four
+// waves 0.17 ms (the pass waits on memory; more bins in flight beat prefetch).
+constexpr int kQuadOffBytes = 1024;
+constexpr int kQuadMtBytes = 2048;
+
+__global__ __launch_bounds__(kBinThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_peel_detect0(
     const qsft_peel_desc* __restrict__ descs, const int32_t* __restrict__ pre, int n_inst) {
     __shared__ double2 tab[QSFT_MAX_Q];
     __shared__ union WaveLds {
         QuadWaveLds quad;
         DetectWaveLds<2> one;
     } wl[kWaves];
-    __shared__ __align__(16) int8_t offbuf[kStageOffBytes];
-    __shared__ __align__(16) int8_t mtbuf[kStageMtBytes];
+    __shared__ __align__(16) int8_t offbuf[kQuadOffBytes];
+    __shared__ __align__(16) int8_t mtbuf[kQuadMtBytes];
     const int inst = find_inst(pre, n_inst, blockIdx.x);
     const qsft_peel_desc& d = descs[inst];
     uint32_t* __restrict__ ctr = d.ctr;
@@ -1353,8 +1391,8 @@ __global__ __launch_bounds__(kBinThreads) __attribute__((amdgpu_waves_per_eu(2))
         return;
     }
     fill_wtable_d(tab, d.q);
-    const bool offs_staged = stage_offsets(d, offbuf);
-    const bool mt_staged = stage_mt(d, mtbuf);
+    const bool offs_staged = stage_offsets<kQuadOffBytes>(d, offbuf);
+    const bool mt_staged = stage_mt<kQuadMtBytes>(d, mtbuf);
     __syncthreads();
     const bool unit = (d.flags & QSFT_PEEL_UNIT_D) != 0;
     // every due bin of the wave written by the assembler (sparse U)
@@ -1559,7 +1597,7 @@ __global__ __launch_bounds__(64 * tail_waves<NR>()) void k_peel_tail(const qsft_
                                                                   int it0, int it_end) {
     constexpr int kW = tail_waves<NR>();
     __shared__ double2 tab[QSFT_MAX_Q];
-    __shared__ DetectWaveLds<NR> wl[kW];
+    __shared__ DetectWaveLds<NR, kBinsPerWave> wl[kW];
     __shared__ __align__(16) int8_t offbuf[kStageOffBytes];
     __shared__ __align__(16) int8_t mtbuf[kStageMtBytes];
     __shared__ uint64_t tdue[kW][kTailBatch];                      // due masks of the wave's scanned groups
