@@ -115,10 +115,37 @@ class Result:
         return {tuple(int(x) for x in row): complex(v) for row, v in zip(self.k, self.values)}
 
 
-class ServedBatch:
-    """One batch of requests on the native runtime: staged, bound, launched."""
+def _arena_take(arena, name, nbytes, device, pinned=False):
+    """A uint8 buffer of at least nbytes from `arena` (a dict kept per server
+    slot), grown by a quarter beyond the request when it is too small; None:
+    a fresh allocation."""
+    nbytes = max(int(nbytes), 256)
+    if arena is not None:
+        buf = arena.get(name)
+        if buf is not None and buf.numel() >= nbytes:
+            return buf[:nbytes]
+        nbytes_alloc = nbytes + nbytes // 4
+    else:
+        nbytes_alloc = nbytes
+    if pinned:
+        buf = torch.empty(nbytes_alloc, dtype=torch.uint8, pin_memory=True)
+    else:
+        buf = torch.empty(nbytes_alloc, dtype=torch.uint8, device=device)
+    if arena is not None:
+        arena[name] = buf
+    return buf[:nbytes]
 
-    def __init__(self, specs, device, stream=None, tail_stream=None, max_iter=15, fuse_from=3, structs=None):
+
+class ServedBatch:
+    """One batch of requests on the native runtime: staged, bound, launched.
+
+    arena: a dict of staging and device buffers reused across batches (the
+    server keeps one per slot; a slot runs one batch at a time and takes the
+    next only once the GPU is done with the last), so a steady serving loop
+    allocates only each batch's output. None: buffers of this batch alone."""
+
+    def __init__(self, specs, device, stream=None, tail_stream=None, max_iter=15, fuse_from=3, structs=None,
+                 arena=None):
         self.specs = list(specs)
         n = len(self.specs)
         if structs is None:
@@ -131,13 +158,13 @@ class ServedBatch:
         self.h = ctypes.c_void_p(h)
         self.sizes = sz
         self.bytes = int(sz.input_bytes + sz.work_bytes + sz.plan_bytes)
-        pin = dict(dtype=torch.uint8, pin_memory=True)
-        self.in_host = torch.empty(max(int(sz.input_bytes), 256), **pin)
-        self.plan_host = torch.empty(max(int(sz.plan_bytes), 256), **pin)
-        self.out = torch.empty(int(sz.out_bytes), **pin)
-        self.in_dev = torch.empty(max(int(sz.input_bytes), 256), dtype=torch.uint8, device=device)
-        self.work_dev = torch.empty(max(int(sz.work_bytes), 256), dtype=torch.uint8, device=device)
-        self.plan_dev = torch.empty(max(int(sz.plan_bytes), 256), dtype=torch.uint8, device=device)
+        self.in_host = _arena_take(arena, "in_host", sz.input_bytes, device, pinned=True)
+        self.plan_host = _arena_take(arena, "plan_host", sz.plan_bytes, device, pinned=True)
+        # the output is handed to the requests (views): never from the arena
+        self.out = torch.empty(int(sz.out_bytes), dtype=torch.uint8, pin_memory=True)
+        self.in_dev = _arena_take(arena, "in_dev", sz.input_bytes, device)
+        self.work_dev = _arena_take(arena, "work_dev", sz.work_bytes, device)
+        self.plan_dev = _arena_take(arena, "plan_dev", sz.plan_bytes, device)
         base = self.in_host.data_ptr()
         # [4][n] addresses of every request's M, D, locq and strengths
         ptrs = np.ascontiguousarray(np.array([sp.wire_ptrs() for sp in self.specs], dtype=np.uint64).T)
@@ -244,7 +271,13 @@ class Server:
 
     hbm_budget: bytes that requests in flight may hold together (admission);
     max_batch: requests per batch; max_inflight: batches on the GPU at once
-    (each on its own stream; finalize and pack on a tail stream per slot)."""
+    (each on its own stream; finalize and pack on a tail stream per slot).
+    A free slot takes whatever waits at once (holding requests back until a
+    batch fills measured no better at 80-90 % load: p99 2.6-3.5 ms either way).
+
+    Each slot keeps its batches' staging and device buffers (grown to the
+    largest batch it has run, outside the budget's count), so a steady loop
+    allocates only each batch's pinned output."""
 
     def __init__(self, hbm_budget, max_batch=64, max_inflight=2, max_iter=15, fuse_from=3, device=None):
         self.device = device if device is not None else torch.device("cuda", torch.cuda.current_device())
@@ -258,6 +291,7 @@ class Server:
         self.streams = [torch.cuda.Stream(device=self.device) for _ in range(self.max_inflight)]
         self.tails = [torch.cuda.Stream(device=self.device) for _ in range(self.max_inflight)]
         self.free_slots = list(range(self.max_inflight))
+        self.arenas = [dict() for _ in range(self.max_inflight)]    # per slot: staging and device buffers
         self.completed = []
         self.batches = 0
         self.host_s = {"submit": 0.0, "build": 0.0, "launch": 0.0, "retire": 0.0}     # host seconds per phase
@@ -322,7 +356,7 @@ class Server:
             t0 = time.perf_counter()
             bt = ServedBatch([r.spec for r in take], self.device, stream=self.streams[slot],
                              tail_stream=self.tails[slot], max_iter=self.max_iter, fuse_from=self.fuse_from,
-                             structs=[r.struct for r in take])
+                             structs=[r.struct for r in take], arena=self.arenas[slot])
             now = time.perf_counter()
             self.host_s["build"] += now - t0
             for r in take:
