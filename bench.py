@@ -397,10 +397,10 @@ def main():
     budget = 200 << 30                                              # HBM the served requests may hold
     rng = np.random.default_rng(args.seed)
     nreq = args.steps * args.batch
-    log(f"generating {nreq} + {nreq} K1 requests", args.quiet)
+    log(f"generating {nreq} + {2 * nreq} K1 requests", args.quiet)
     warm = k1_specs(args.warmup * args.batch, rng)
     sat_specs = k1_specs(nreq, rng)
-    open_specs = k1_specs(nreq, rng)
+    open_specs = k1_specs(2 * nreq, rng)                          # twice as long: a steadier p99
 
     # ---- headline: the serving loop at saturation, then open loop at 80 %
     serve_closed(warm, args.batch, args.inflight, budget)
@@ -472,7 +472,8 @@ def main():
                    "parallelism": "continuous batching on 1 GPU"},
         "latency_ms": {"p50": openl["p50_ms"], "p99": openl["p99_ms"],
                        "of": "per request, arrival -> result in host memory, open-loop Poisson arrivals at 80% of "
-                             "the saturation rate, after 256 warm-up requests at that rate",
+                             "the saturation rate, after 256 warm-up requests at that rate; twice the saturation run's "
+                             "request count",
                        "open_loop": openl, "saturation_p50": sat_lat["p50_ms"], "saturation_p99": sat_lat["p99_ms"]},
         "serving_host_us_per_request": host_us,
         "plain_fp64": plain,
