@@ -704,7 +704,7 @@ __device__ __forceinline__ void eval_mfma_body(const qsft_eval_mfma_desc* __rest
                     const v4f v = odd ? v4f{o[2] * unscale, a[2] * unscale, o[3] * unscale, a[3] * unscale}
                                       : v4f{a[0] * unscale, o[0] * unscale, a[1] * unscale, o[1] * unscale};
                     if (even_q) {                                  // l0 even, so l0 + 1 < Q
-                        *reinterpret_cast<QSFT_GLOBAL v4f*>(row) = v;
+                        __builtin_nontemporal_store(v, reinterpret_cast<QSFT_GLOBAL v4f*>(row));   // streamed out
                     } else {
                         *reinterpret_cast<QSFT_GLOBAL v2f*>(row) = v2f{v[0], v[1]};
                         if (l0 + 1 < Q) *reinterpret_cast<QSFT_GLOBAL v2f*>(row + 2) = v2f{v[2], v[3]};
