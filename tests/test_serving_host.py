@@ -28,3 +28,17 @@ def test_collect_young_keeps_generation_zero_small():
         SV.collect_young()
         assert gc.get_count()[0] <= SV.GC_YOUNG
         del junk
+
+
+def test_arena_reuses_and_grows_slot_buffers():
+    import torch
+    arena = {}
+    a = SV._arena_take(arena, "work", 1000, torch.device("cpu"))
+    assert a.numel() == 1000 and arena["work"].numel() >= 1000
+    base = arena["work"].data_ptr()
+    b = SV._arena_take(arena, "work", 800, torch.device("cpu"))       # fits: the same storage, a view
+    assert b.numel() == 800 and b.data_ptr() == base
+    c = SV._arena_take(arena, "work", 5000, torch.device("cpu"))      # too small: replaced, with headroom
+    assert c.numel() == 5000 and arena["work"].numel() >= 5000 + 5000 // 4
+    d = SV._arena_take(None, "work", 10, torch.device("cpu"))         # no arena: a fresh buffer of at least 256 B
+    assert d.numel() == 256
